@@ -1,0 +1,62 @@
+"""Build step for the native parts of qubo_lib_amd (run by __graft_entry__.build()).
+
+  libqbx.so      hipcc, gfx950: kernels + C ABI (include/qbx.h)  -> qubo_lib_amd/libqbx.so
+  libqbx_emu.so  g++: host build of the same lane walk, loaded only by the CPU test
+                 suite to check the walk's indexing without a GPU -> tests/_build/
+
+Every call recompiles.  Outputs are git-ignored; they travel to the GPU box with the tree.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent
+CSRC = HERE / "csrc"
+INCLUDE = REPO / "include"
+LIB_OUT = HERE / "qubo_lib_amd" / "libqbx.so"
+EMU_OUT = REPO / "tests" / "_build" / "libqbx_emu.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 (v_pk_*_f32) stays off: the kernels must not depend on it
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed ({r.returncode}): {' '.join(map(str, cmd))}")
+    return r
+
+
+def build_lib(out: Path = LIB_OUT) -> Path:
+    out.parent.mkdir(parents=True, exist_ok=True)
+    tmp = out.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "qbx.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_emu(out: Path = EMU_OUT) -> Path:
+    out.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{CSRC}", "-o", str(out),
+          str(CSRC / "qbx_emu.cpp")])
+    return out
+
+
+def build_all() -> list[Path]:
+    return [build_lib(), build_emu()]
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)

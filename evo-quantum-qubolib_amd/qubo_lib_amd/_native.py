@@ -1,0 +1,229 @@
+"""ctypes binding of libqbx.so, the gfx950 exhaustive-QUBO library (C ABI in include/qbx.h).
+
+This module is the only place the product touches native code.  There is no CPU
+fallback: if the library is missing or no GPU is visible, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo",
+    "load_library", "get_context",
+]
+
+LIB_PATH = Path(__file__).resolve().parent / "libqbx.so"
+
+QBX_OK = 0
+QBX_EINVAL = -1
+QBX_ERANGE = -2
+QBX_EDEVICE = -3
+QBX_ENOMEM = -4
+QBX_EBUSY = -5
+
+
+class NativeError(RuntimeError):
+    """A libqbx.so call failed (device, memory or range error)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"libqbx error {code}: {message}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [
+        ("tie_B", ctypes.c_int32),
+        ("walk_bits", ctypes.c_int32),
+        ("inner_bits", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("E_int", ctypes.c_int64),
+        ("E", ctypes.c_double),
+        ("state", ctypes.c_uint64),
+        ("rank", ctypes.c_uint64),
+        ("states", ctypes.c_uint64),
+        ("tiles", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
+        ("n", ctypes.c_int32),
+        ("tie_B", ctypes.c_int32),
+        ("walk_bits", ctypes.c_int32),
+        ("inner_bits", ctypes.c_int32),
+        ("lane_bits", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class DevInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("compute_units", ctypes.c_int32),
+        ("clock_khz", ctypes.c_int32),
+        ("simd_per_cu", ctypes.c_int32),
+        ("wave_size", ctypes.c_int32),
+        ("total_mem", ctypes.c_int64),
+        ("valu_peak_ops", ctypes.c_double),
+    ]
+
+
+# every symbol declared in include/qbx.h, with its ctypes signature
+_P = ctypes.POINTER
+_SIGNATURES = {
+    "qbx_open": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_void_p)]),
+    "qbx_close": (ctypes.c_int, [ctypes.c_void_p]),
+    "qbx_last_error": (ctypes.c_char_p, []),
+    "qbx_device_info": (ctypes.c_int, [ctypes.c_void_p, _P(DevInfo)]),
+    "qbx_solve_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_solve_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_batch_i32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_int32), _P(ctypes.c_int64),
+                                     _P(ctypes.c_int32), _P(Opts), _P(Result)]),
+    "qbx_batch_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
+                                     _P(ctypes.c_int32), _P(Opts), _P(Result)]),
+}
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load libqbx.so (built in-tree by __graft_entry__.build()).  Raises if it is missing."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise NativeError(QBX_EDEVICE, f"{p} not found: build it with __graft_entry__.build()")
+        lib = ctypes.CDLL(str(p))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def _check(lib, rc: int):
+    if rc == QBX_OK:
+        return
+    msg = lib.qbx_last_error().decode(errors="replace")
+    if rc == QBX_EINVAL:
+        raise ValueError(msg)
+    if rc == QBX_ERANGE:
+        raise OverflowError(msg)
+    raise NativeError(rc, msg)
+
+
+def _opts(tie_B=None, walk_bits=None, inner_bits=None) -> Opts:
+    return Opts(-1 if tie_B is None else int(tie_B),
+                -1 if walk_bits is None else int(walk_bits),
+                -1 if inner_bits is None else int(inner_bits), 0)
+
+
+class Context:
+    """One HIP device and stream.  Not thread-safe: one host thread at a time."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self._h = ctypes.c_void_p()
+        _check(self.lib, self.lib.qbx_open(int(device), ctypes.byref(self._h)))
+        self.device = int(device)
+        self.lock = threading.Lock()
+
+    def close(self):
+        if self._h:
+            self.lib.qbx_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self) -> dict:
+        info = DevInfo()
+        _check(self.lib, self.lib.qbx_device_info(self._h, ctypes.byref(info)))
+        d = {name: getattr(info, name) for name, _ in DevInfo._fields_}
+        d["name"] = info.name.decode()
+        return d
+
+    # -- single problems ------------------------------------------------------------
+    def solve_i32(self, Q: np.ndarray, **kw) -> Result:
+        Q = np.ascontiguousarray(Q, dtype=np.int32)
+        n = Q.shape[0]
+        out = Result()
+        o = _opts(**kw)
+        with self.lock:
+            rc = self.lib.qbx_solve_i32(self._h, Q.ctypes.data_as(_P(ctypes.c_int32)), n,
+                                        ctypes.byref(o), ctypes.byref(out))
+        _check(self.lib, rc)
+        return out
+
+    def solve_f64(self, Q: np.ndarray, **kw) -> Result:
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        n = Q.shape[0]
+        out = Result()
+        o = _opts(**kw)
+        with self.lock:
+            rc = self.lib.qbx_solve_f64(self._h, Q.ctypes.data_as(_P(ctypes.c_double)), n,
+                                        ctypes.byref(o), ctypes.byref(out))
+        _check(self.lib, rc)
+        return out
+
+    # -- batches --------------------------------------------------------------------
+    @staticmethod
+    def pack(Qs, dtype):
+        ns = np.array([q.shape[0] for q in Qs], dtype=np.int32)
+        sizes = ns.astype(np.int64) ** 2
+        offsets = np.zeros(len(Qs), dtype=np.int64)
+        if len(Qs) > 1:
+            offsets[1:] = np.cumsum(sizes)[:-1]
+        packed = np.empty(int(sizes.sum()), dtype=dtype)
+        for q, o, s in zip(Qs, offsets, sizes):
+            packed[o:o + s] = np.asarray(q, dtype=dtype).ravel()
+        return packed, offsets, ns
+
+    def batch(self, Qs, kind: str = "i32", packed=None, **kw):
+        """Solve a list of problems in one persistent launch; returns a list of Result."""
+        dtype, fn, ptr = ((np.int32, self.lib.qbx_batch_i32, ctypes.c_int32) if kind == "i32"
+                          else (np.float64, self.lib.qbx_batch_f64, ctypes.c_double))
+        if packed is None:
+            packed = self.pack(Qs, dtype)
+        data, offsets, ns = packed
+        count = len(ns)
+        out = (Result * count)()
+        o = _opts(**kw)
+        with self.lock:
+            rc = fn(self._h, count, data.ctypes.data_as(_P(ptr)), offsets.ctypes.data_as(_P(ctypes.c_int64)),
+                    ns.ctypes.data_as(_P(ctypes.c_int32)), ctypes.byref(o), out)
+        if rc not in (QBX_OK, QBX_ERANGE):
+            _check(self.lib, rc)
+        return list(out)
+
+
+_contexts: dict[int, Context] = {}
+_ctx_lock = threading.Lock()
+
+
+def get_context(device: int = 0) -> Context:
+    """Process-wide context per HIP device (opened on first use)."""
+    with _ctx_lock:
+        c = _contexts.get(device)
+        if c is None:
+            c = Context(device)
+            _contexts[device] = c
+        return c

@@ -1,0 +1,107 @@
+/*
+ * qbx.h -- C ABI of libqbx.so, the gfx950 (MI355X) exhaustive QUBO solver.
+ *
+ * Every entry point returns 0 on success and a negative QBX_E* code on failure; the
+ * message of the last failure on the calling thread is returned by qbx_last_error().
+ * No exception crosses this boundary.  Input pointers are host memory owned by the
+ * caller and read only during the call.  Output structs are caller-allocated.  Device
+ * memory belongs to the context.  One context is used by one host thread at a time.
+ *
+ * Conventions shared with the reference (src/qubo_lib/solvers/qbf.py):
+ *   - Q is n x n, row-major, any (possibly non-symmetric) matrix; E(x) = x^T Q x
+ *     (qbf.py:264, qbf.py:499).
+ *   - The state integer has x[0] as its most significant bit (qbf.py:373-393).
+ *   - Ties are broken exactly as the reference QBF breaks them with suffix width
+ *     B = tie_B: the answer is the minimum of (E, rank) with
+ *     rank(s) = (invgray(s >> B) << B) | (s & (2^B - 1))  (qbf.py:212-230, 395-412),
+ *     and tie_B = -1 selects the reference CPU default B = min(n, 20) (qbf.py:443-446).
+ */
+#ifndef QBX_H
+#define QBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QBX_OK 0
+#define QBX_EINVAL (-1)     /* bad argument (ValueError on the Python side) */
+#define QBX_ERANGE (-2)     /* energies do not fit the selected arithmetic */
+#define QBX_EDEVICE (-3)    /* HIP runtime error */
+#define QBX_ENOMEM (-4)     /* allocation failure */
+#define QBX_EBUSY (-5)      /* server queue full / budget exhausted */
+
+typedef struct qbx_ctx qbx_ctx_t;
+
+typedef struct {
+    int32_t tie_B;        /* reference suffix width B for the tie rule; -1 = min(n, 20) */
+    int32_t walk_bits;    /* bits walked by one lane (L); -1 = tuned default */
+    int32_t inner_bits;   /* unrolled inner block bits (M); -1 = default (7 int, 6 float) */
+    int32_t flags;        /* reserved, must be 0 */
+} qbx_opts_t;
+
+typedef struct {
+    int64_t E_int;        /* exact minimum energy (integer solves) */
+    double E;             /* minimum energy (all solves; exact FP64 sum for float ones) */
+    uint64_t state;       /* argmin state integer, x[0] = most significant bit */
+    uint64_t rank;        /* rank of `state` under the tie rule */
+    uint64_t states;      /* states enumerated (2^n) */
+    uint64_t tiles;       /* work tiles (64 lanes each) the solve was cut into */
+    double kernel_ms;     /* device time of the solve's kernels (HIP events); batch: whole launch */
+    int32_t n;
+    int32_t tie_B;
+    int32_t walk_bits;
+    int32_t inner_bits;
+    int32_t lane_bits;
+    int32_t status;       /* 0 = ok, QBX_ERANGE = energy bound outside the int32 path */
+} qbx_result_t;
+
+typedef struct {
+    char name[64];        /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t compute_units;
+    int32_t clock_khz;    /* peak engine clock */
+    int32_t simd_per_cu;  /* 4 on CDNA */
+    int32_t wave_size;    /* 64 */
+    int64_t total_mem;    /* bytes */
+    double valu_peak_ops; /* CUs * SIMDs * 16 lanes * clock: int32 VALU lane-ops per second */
+} qbx_devinfo_t;
+
+/* ------------------------------------------------------------------ context */
+/* Opens HIP device `device` and a stream of default priority on it. */
+int qbx_open(int device, qbx_ctx_t **ctx);
+int qbx_close(qbx_ctx_t *ctx);
+const char *qbx_last_error(void);
+int qbx_device_info(qbx_ctx_t *ctx, qbx_devinfo_t *info);
+
+/* ------------------------------------------------------------ single solves
+ * Replaces the reference's hot loop QBF.iterate_prefix_interval (qbf.py:188-230) and the
+ * QB_F1 kernel launch (qbf1.py:69-77, kernel qbf1.py:179-249), together with the caches
+ * they need (qbf.py:246-289; qbf1.py:105-118), for one problem.
+ *
+ * qbx_solve_i32: exact int32 arithmetic.  Requires
+ *   bound = sum_i |Q_ii| + sum_{i<j} |Q_ij + Q_ji| <= 2^31 - 2, else QBX_ERANGE.
+ * qbx_solve_f64: FP32 block filter + FP64 exact re-walk; Q given as doubles (float32 and
+ *   integer inputs convert exactly).  Integer-valued Q gives exact results while
+ *   bound < 2^53.  1 <= n <= 56 for both. */
+int qbx_solve_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opts,
+                  qbx_result_t *out);
+int qbx_solve_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts,
+                  qbx_result_t *out);
+
+/* ------------------------------------------------------------------ batches
+ * `count` independent problems solved in one persistent launch (plus one table-building
+ * launch).  Q_i is stored row-major at Qpacked + offsets[i] with n = ns[i].  Replaces a
+ * loop of QBF().solve calls, i.e. the per-task do_solve of the reference's Parallel
+ * bruteforce path (src/qubo_lib/solvers/parallel.py:145-157, 381-385), for problems
+ * that each fit one GPU.  out[i].kernel_ms is the time of the whole launch. */
+int qbx_batch_i32(qbx_ctx_t *ctx, int count, const int32_t *Qpacked, const int64_t *offsets,
+                  const int32_t *ns, const qbx_opts_t *opts, qbx_result_t *out);
+int qbx_batch_f64(qbx_ctx_t *ctx, int count, const double *Qpacked, const int64_t *offsets,
+                  const int32_t *ns, const qbx_opts_t *opts, qbx_result_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QBX_H */
