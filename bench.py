@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: exhaustive QUBO solve throughput on one MI355X.
+
+Default workload (the headline metric): RandomQUBO-style n = 40, Q = triu(U) + triu(U,1)^T
+with U ~ integers[-127, 127] int16 (PCG64 seed 0), all 2^40 states enumerated exactly with
+int32 accumulation.  One step = one complete solve (prep + walk kernels) of that instance;
+Q is staged once per solve (3.2 KB) and the answer (16 B) read back.
+
+    python bench.py [--steps K] [--warmup W] [--workload exhaustive|float|batch]
+                    [--no-cpu-baseline]
+
+Prints ONE JSON line.  `value` is states/s for the whole job (one GPU).  `roofline` prices
+the walk kernel against the gfx950 int32 VALU peak (CUs x 4 SIMD x 16 lanes x clock) with
+2 algorithmic ops per state (one energy update, one comparison); `cpu_baseline` is the
+numpy restatement of the reference QBF algorithm (oracle/qbf_numpy.py) timed on a bounded
+sample of the same instance on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "evo-quantum-qubolib_amd"))
+
+import numpy as np  # noqa: E402
+
+OPS_PER_STATE = 2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(Q, seconds_target: float = 12.0) -> dict:
+    """Numpy restatement of QBF on the first K prefix steps of the same instance."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle.qbf_numpy import QBFNumpy
+    q = QBFNumpy(Q)
+    t0 = time.perf_counter()
+    q.build_cache()
+    build_s = time.perf_counter() - t0
+    # calibrate on 16 steps, then run a sample of about `seconds_target`
+    _, _, done, sec = q.run(steps=16)
+    per_step = sec / max(done, 1)
+    steps = int(max(16, min(2 ** q.A, seconds_target / max(per_step, 1e-6))))
+    _, _, done, sec = q.run(steps=steps)
+    states = done * 2 ** q.B
+    return {
+        "value": states / sec,
+        "unit": "states/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"numpy QBF restatement (oracle/qbf_numpy.py): first {done} of 2^{q.A} Gray prefix steps "
+                   f"of the same n={Q.shape[0]} instance, {states:.3e} states in {sec:.2f} s; "
+                   f"cache build ({build_s:.1f} s) excluded; host cores available: {os.cpu_count()}"),
+    }
+
+
+def exhaustive(args, ctx, devinfo):
+    from qubo_lib_amd import engine
+    from qubo_lib_amd.problems.generators import symmetric_int_qubo, symmetric_normal_qubo
+    if args.workload == "float":
+        n = 34
+        Q = symmetric_normal_qubo(n, args.seed)
+        workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: FP32 filter walk, FP64 exact re-walk"
+        dtype = "f32"
+    else:
+        n = 40
+        Q = symmetric_int_qubo(n, args.seed, -127, 127, np.int16)
+        workload = f"RandomQUBO n={n}, symmetric int16 U[-127,127] (seed {args.seed}), 2^{n} states, exact"
+        dtype = "int32"
+    prep = engine.prepare(Q)
+    kw = dict(walk_bits=args.walk_bits, inner_bits=args.inner_bits)
+    for _ in range(args.warmup):
+        engine.solve_matrix(prep, **kw)
+    kms, results = [], []
+    # every libqbx call ends with hipStreamSynchronize on its stream, so the clock below
+    # brackets completed device work (torch is not loaded: it carries its own HIP runtime)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = engine.solve_matrix(prep, **kw)
+        kms.append(r.info["kernel_ms"])
+        results.append((r.E, r.state))
+    wall = time.perf_counter() - t0
+    states = 2.0 ** n
+    ms_per_step = wall * 1000.0 / args.steps
+    value = states * args.steps / wall
+    kernel_ms = float(np.mean(kms))
+    peak = devinfo["valu_peak_ops"]
+    achieved = states * OPS_PER_STATE / (kernel_ms * 1e-3)
+    assert len(set(results)) == 1, "answers differ between steps"
+    E, state = results[0]
+    parity = {"E": E, "state": state}
+    gold = ROOT / "tests" / "golden" / "oracle_goldens.json"
+    key = f"sym_int16_n40_s{args.seed}" if args.workload != "float" else f"sym_normal_n34_s{args.seed}"
+    if gold.exists():
+        g = json.loads(gold.read_text()).get(key)
+        if g is not None:
+            parity["oracle_state"] = g["state"]
+            parity["matches_oracle"] = bool(g["state"] == state and
+                                            (g["E"] == E if isinstance(E, int) else abs(g["E"] - E) < 1e-9 * max(1, abs(E))))
+    line = {
+        "metric": "states/s at n=40 int16 (exact)" if args.workload != "float" else "states/s at n=34 float32 (FP64-exact argmin)",
+        "value": value,
+        "unit": "states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": workload, "n": n, "seed": args.seed, "states_per_step": int(states),
+                   "walk_bits": r.info["walk_bits"], "inner_bits": r.info["inner_bits"],
+                   "lane_bits": r.info["lane_bits"], "tiles": r.info["tiles"], "tie_B": r.tie_B,
+                   "kernel_ms_mean": kernel_ms, "device": devinfo["name"], "compute_units": devinfo["compute_units"],
+                   "clock_mhz": devinfo["clock_khz"] / 1000.0},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "ops_per_state": OPS_PER_STATE,
+                     "note": "kernel time = HIP events around prep+walk on the library stream"},
+        "parity": parity,
+    }
+    if args.cpu_baseline:
+        log("timing the CPU baseline ...")
+        line["cpu_baseline"] = cpu_baseline(Q)
+    else:
+        line["cpu_baseline"] = None
+    return line
+
+
+def batch(args, ctx, devinfo):
+    """4096-problem uneven mix: one persistent launch vs one launch per problem."""
+    from qubo_lib_amd import engine
+    from qubo_lib_amd.problems.generators import batch_mix
+    probs, kinds = batch_mix(args.batch_count, seed=args.seed)
+    Qs = [p.build_Q_matrix() for p in probs]
+    preps = [engine.prepare(q) for q in Qs]
+    mats = [p.matrix for p in preps]
+    packed = ctx.pack(mats, np.int32)
+    for _ in range(args.warmup):
+        ctx.batch(mats, kind="i32", packed=packed)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = ctx.batch(mats, kind="i32", packed=packed)
+    wall_b = (time.perf_counter() - t0) / args.steps
+    # plain form: one C-ABI solve (launch + sync) per problem, same kernels and GPU
+    lat = []
+    t0 = time.perf_counter()
+    plain = []
+    for m in mats:
+        t1 = time.perf_counter()
+        plain.append(ctx.solve_i32(m, walk_bits=None))
+        lat.append(time.perf_counter() - t1)
+    wall_p = time.perf_counter() - t0
+    same = all((a.E_int, a.state) == (b.E_int, b.state) for a, b in zip(res, plain))
+    states = float(sum(2.0 ** q.shape[0] for q in Qs))
+    return {
+        "metric": "batched solves/s (4096-instance uneven mix)",
+        "value": len(Qs) / wall_b,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall_b * 1000.0,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"batch_mix({len(Qs)}, seed={args.seed}): MaxCut G(n,0.5) n~U{{16..32}} / int TSP 4-5 cities",
+                   "states": states},
+        "plain": {"solves_per_s": len(Qs) / wall_p, "wall_s": wall_p, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                  "p99_ms": float(np.percentile(lat, 99) * 1e3)},
+        "speedup_vs_plain": wall_p / wall_b,
+        "identical_answers": same,
+        "states_per_s": states / wall_b,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["exhaustive", "float", "batch"], default="exhaustive")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--walk-bits", type=int, default=None)
+    ap.add_argument("--inner-bits", type=int, default=None)
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the prefix space would shard across GPUs with one 8-byte min-reduction; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    from qubo_lib_amd import _native
+    ctx = _native.get_context(0)
+    devinfo = ctx.device_info()
+    if args.workload == "batch":
+        line = batch(args, ctx, devinfo)
+    else:
+        line = exhaustive(args, ctx, devinfo)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""Host-side dispatch from a dense Q to the gfx950 kernels (libqbx.so).
+
+Arithmetic path per problem:
+  * integer Q with energy bound  sum|Q_ii| + sum_{i<j}|Q_ij + Q_ji|  <= 2^31 - 2:
+      exact int32 walk (qbx_solve_i32).  Q is passed in folded upper-triangular form
+      (Q'_ij = Q_ij + Q_ji, i < j), which has the same energies and fits int32 whenever
+      the bound does.
+  * integer Q with bound < 2^53: the FP64 walk, exact on integers (qbx_solve_f64).
+  * float Q: the FP64 walk on the float64 values of Q (an FP32 block filter, every
+      candidate block re-walked in FP64).
+  * anything else: OverflowError.  There is no CPU path: without libqbx.so or a GPU
+    every call raises.
+
+Energies of integer problems come back as Python ints, of float problems as floats.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+MAX_N = 56
+INT32_BOUND = (1 << 31) - 2
+EXACT_F64_BOUND = 1 << 53
+
+
+@dataclass
+class Prepared:
+    """A problem matrix validated and converted for one kernel path."""
+    n: int
+    kind: str                 # 'i32' | 'f64'
+    integer: bool             # the problem's values are integers
+    matrix: np.ndarray        # int32 (folded) or float64
+    bound: float
+
+
+@dataclass
+class RawResult:
+    E: object                 # int for integer problems, float otherwise
+    state: int
+    rank: int
+    n: int
+    tie_B: int
+    kind: str
+    info: dict = field(default_factory=dict)
+
+
+def default_tie_B(n: int) -> int:
+    """The reference CPU QBF's suffix width B = min(N, 20) (qbf.py:443-446)."""
+    return min(n, 20)
+
+
+def prepare(Q) -> Prepared:
+    Q = np.asarray(Q)
+    if Q.ndim != 2 or Q.shape[0] != Q.shape[1]:
+        raise ValueError(f"Q must be a square matrix, got shape {Q.shape}")
+    n = Q.shape[0]
+    if n < 1 or n > MAX_N:
+        raise ValueError(f"exhaustive solve needs 1 <= N <= {MAX_N}, got N={n}")
+    if Q.dtype == np.bool_:
+        Q = Q.astype(np.int64)
+    if np.issubdtype(Q.dtype, np.integer):
+        Qo = Q.astype(object)
+        up = np.triu(Qo, 1) + np.triu(Qo.T, 1)          # Q_ij + Q_ji for i < j, exact
+        diag = np.diag(Qo)
+        bound = int(sum(abs(v) for v in diag)) + int(sum(abs(v) for v in up.ravel()))
+        folded = up + np.diag(diag)
+        if bound <= INT32_BOUND:
+            return Prepared(n, "i32", True, np.ascontiguousarray(folded.astype(np.int64).astype(np.int32)), bound)
+        if bound < EXACT_F64_BOUND:
+            return Prepared(n, "f64", True, np.ascontiguousarray(folded.astype(np.float64)), bound)
+        raise OverflowError(f"integer Q with energy bound {bound} >= 2^53 cannot be solved exactly")
+    if np.issubdtype(Q.dtype, np.floating):
+        Qf = Q.astype(np.float64)
+        if not np.all(np.isfinite(Qf)):
+            raise ValueError("Q contains non-finite values")
+        bound = float(np.abs(np.diag(Qf)).sum() + np.abs(np.triu(Qf, 1) + np.triu(Qf.T, 1)).sum())
+        return Prepared(n, "f64", False, np.ascontiguousarray(Qf), bound)
+    raise TypeError(f"unsupported Q dtype {Q.dtype}")
+
+
+def _finish(p: Prepared, r: _native.Result) -> RawResult:
+    if p.integer:
+        E = int(r.E_int) if p.kind == "i32" else int(round(r.E))
+    else:
+        E = float(r.E)
+    info = dict(kernel_ms=r.kernel_ms, states=int(r.states), tiles=int(r.tiles), walk_bits=r.walk_bits,
+                inner_bits=r.inner_bits, lane_bits=r.lane_bits, path=p.kind, bound=p.bound)
+    return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=p.n, tie_B=r.tie_B, kind=p.kind, info=info)
+
+
+def solve_matrix(Q, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None) -> RawResult:
+    """Exact minimum of x^T Q x and its argmin under the reference's tie rule."""
+    p = Q if isinstance(Q, Prepared) else prepare(Q)
+    if tie_B is not None and not (0 <= int(tie_B) <= p.n):
+        raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
+    ctx = _native.get_context(device)
+    kw = dict(tie_B=tie_B, walk_bits=walk_bits, inner_bits=inner_bits)
+    r = ctx.solve_i32(p.matrix, **kw) if p.kind == "i32" else ctx.solve_f64(p.matrix, **kw)
+    return _finish(p, r)
+
+
+def solve_matrices(Qs, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None):
+    """Many problems, one persistent launch per arithmetic path.  Returns RawResults in
+    input order.  `tie_B` may be a single value (None = min(N, 20) per problem)."""
+    preps = [q if isinstance(q, Prepared) else prepare(q) for q in Qs]
+    out = [None] * len(preps)
+    ctx = _native.get_context(device)
+    for kind in ("i32", "f64"):
+        idx = [i for i, p in enumerate(preps) if p.kind == kind]
+        if not idx:
+            continue
+        res = ctx.batch([preps[i].matrix for i in idx], kind=kind, tie_B=tie_B, walk_bits=walk_bits,
+                        inner_bits=inner_bits)
+        for i, r in zip(idx, res):
+            if r.status != 0:
+                raise OverflowError(f"problem {i}: energy bound outside the {kind} path")
+            out[i] = _finish(preps[i], r)
+    return out
+
+
+def state_bits(state: int, n: int) -> np.ndarray:
+    """State integer -> int32 array of n bits, x[0] first (qbf.py:387-393)."""
+    return np.array([(state >> (n - 1 - i)) & 1 for i in range(n)], dtype=np.int32)

@@ -1,0 +1,59 @@
+"""Shared test helpers: small independent checks and instance makers."""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+
+def invgray(v: int) -> int:
+    r = 0
+    while v:
+        r ^= v
+        v >>= 1
+    return r
+
+
+def rank_of(s: int, B: int) -> int:
+    return (invgray(s >> B) << B) | (s & ((1 << B) - 1))
+
+
+def itertools_min(Q, B=None):
+    """Pure-Python enumeration of all 2^n states, exact (E, rank) minimum (n <= 12)."""
+    Q = np.asarray(Q)
+    n = Q.shape[0]
+    B = min(n, 20) if B is None else B
+    best = None
+    for bits in itertools.product((0, 1), repeat=n):
+        x = np.array(bits)
+        E = x @ Q @ x
+        E = int(E) if np.issubdtype(Q.dtype, np.integer) else float(E)
+        s = int("".join(map(str, bits)), 2) if n else 0
+        key = (E, rank_of(s, B))
+        if best is None or key < best[0]:
+            best = (key, s)
+    return best[0][0], best[1]
+
+
+def random_instance(rng, n, kind):
+    if kind == "uniform":
+        return rng.integers(-127, 128, (n, n))
+    if kind == "maxcut":
+        Q = np.zeros((n, n), dtype=np.int64)
+        for u, v in zip(*np.nonzero(np.triu(rng.random((n, n)) < 0.5, 1))):
+            Q[u, u] -= 1
+            Q[v, v] -= 1
+            Q[u, v] += 2
+        return Q
+    if kind == "ternary":
+        return rng.integers(-1, 2, (n, n))
+    if kind == "sparse":
+        return rng.integers(-5, 6, (n, n)) * (rng.random((n, n)) < 0.3)
+    if kind == "zero":
+        return np.zeros((n, n), dtype=np.int64)
+    if kind == "positive":
+        return rng.integers(1, 10, (n, n))
+    raise ValueError(kind)
+
+
+KINDS = ("uniform", "maxcut", "ternary", "sparse", "zero", "positive")

@@ -1,0 +1,83 @@
+"""Host-side logic that runs without a GPU: arithmetic-path selection, argument checks,
+the reference's A/B split rules."""
+import numpy as np
+import pytest
+
+from qubo_lib_amd import engine
+from qubo_lib_amd.problems import RandomQUBO
+from qubo_lib_amd.solvers import QB_F1, QBF, Parallel
+from qubo_lib_amd.solvers.qbf import adjust_AB_to_fit_memory, define_chunk_size
+
+
+def test_path_selection():
+    p = engine.prepare(np.array([[1, 2], [3, 4]], dtype=np.int16))
+    assert p.kind == "i32" and p.integer and p.bound == 1 + 4 + 5
+    assert p.matrix.tolist() == [[1, 5], [0, 4]]          # folded upper triangle
+    big = np.array([[2 ** 40, 0], [0, 1]], dtype=np.int64)
+    p = engine.prepare(big)
+    assert p.kind == "f64" and p.integer
+    # huge entries that cancel in Q_ij + Q_ji still fit the int32 path
+    c = np.array([[0, 2 ** 40], [-(2 ** 40), 0]], dtype=np.int64)
+    assert engine.prepare(c).kind == "i32"
+    with pytest.raises(OverflowError):
+        engine.prepare(np.array([[2 ** 62, 0], [0, 2 ** 62]], dtype=np.int64))
+    f = engine.prepare(np.eye(3, dtype=np.float32))
+    assert f.kind == "f64" and not f.integer
+    with pytest.raises(ValueError):
+        engine.prepare(np.array([[np.nan]]))
+    with pytest.raises(ValueError):
+        engine.prepare(np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        engine.prepare(np.zeros((57, 57)))
+    assert engine.prepare(np.ones((2, 2), dtype=bool)).kind == "i32"
+
+
+def test_reference_split_rules():
+    assert define_chunk_size(30) == (10, 20)
+    assert define_chunk_size(12) == (0, 12)
+    assert define_chunk_size(30, A=4) == (4, 26)
+    assert define_chunk_size(30, B=7) == (23, 7)
+    # qbf.py:455-466: largest B <= N-2 with (A + B) * 2^B * sizeof <= mem
+    A, B = adjust_AB_to_fit_memory(30, 1.0, 4)
+    assert A + B == 30 and (A + B) * 2 ** B * 4 <= 2 ** 30 < (30) * 2 ** (B + 1) * 4
+    assert define_chunk_size(30, use_mem_GB=1.0, size_of=4) == (A, B)
+
+
+def test_solver_argument_errors():
+    prob = RandomQUBO.init_random_instance(seed=1, N=6, dtype=np.int16)
+    with pytest.raises(ValueError, match="obsolete"):
+        QBF().solve(prob, use_gpu=True)
+    with pytest.raises(ValueError, match="obsolete"):
+        QBF().solve(prob, gpu_id=0)
+    with pytest.raises(ValueError, match="device"):
+        QBF().solve(prob, devices=["cuda"])
+    with pytest.raises(ValueError, match="GPU"):
+        QBF().solve(prob, device="cpu")
+    with pytest.raises(ValueError, match="Unknown device"):
+        QBF().solve(prob, device="tpu")
+    with pytest.raises(ValueError, match="GPU only"):
+        QB_F1().solve(prob, device="cpu")
+    with pytest.raises(TypeError):
+        QB_F1().solve(RandomQUBO.init_random_instance(seed=1, N=6, dtype=np.int32))
+
+
+def test_parallel_validation():
+    with pytest.raises(ValueError, match="power of 2"):
+        Parallel(QBF, 3, ["cuda"])
+    with pytest.raises(ValueError, match="positive"):
+        Parallel(QBF, 0, ["cuda"])
+    with pytest.raises(ValueError, match="empty"):
+        Parallel(QBF, 2, [])
+    with pytest.raises(ValueError, match="mix"):
+        Parallel(QBF, 2, ["cpu", "cuda"])
+    with pytest.raises(ValueError, match="GPUs only"):
+        Parallel(QBF, 2, ["cpu"] * 2)
+    with pytest.raises(ValueError, match="Unsupported"):
+        Parallel(object, 2, ["cuda"])
+    p = Parallel(QBF, 4, ["cuda"] * 2)
+    assert p._devices == ["cuda:0", "cuda:1"]
+
+
+def test_state_bits():
+    assert engine.state_bits(0b1011, 4).tolist() == [1, 0, 1, 1]
+    assert engine.state_bits(1, 3).dtype == np.int32

@@ -1,0 +1,101 @@
+"""The kernels' lane walk (csrc/qbx_walk.h), compiled for the host, against the oracle.
+
+libqbx_emu.so runs the same walk_lane_* templates the gfx950 kernels run, lane by lane,
+on the same bit-ordered tables.  Every (n, walk bits L, tie width B, inner bits M) shape
+is checked here on the CPU, so lane prefixes, block keys, tie ranks and outer transitions
+are pinned before the GPU runs them.
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import KINDS, itertools_min, random_instance
+
+EMU = Path(__file__).resolve().parent / "_build" / "libqbx_emu.so"
+P = ctypes.POINTER
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not EMU.exists():
+        import build_native
+        build_native.build_emu()
+    L = ctypes.CDLL(str(EMU))
+    L.qbx_emu_solve_i32.argtypes = [P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    P(ctypes.c_int64), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    L.qbx_emu_solve_f64.argtypes = [P(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    L.qbx_emu_state_to_rank.restype = ctypes.c_uint64
+    L.qbx_emu_state_to_rank.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    L.qbx_emu_rank_to_state.restype = ctypes.c_uint64
+    L.qbx_emu_rank_to_state.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    return L
+
+
+def run_i32(emu, Q, B, L, M):
+    Q = np.ascontiguousarray(Q, dtype=np.int32)
+    E, r, s = ctypes.c_int64(), ctypes.c_uint64(), ctypes.c_uint64()
+    rc = emu.qbx_emu_solve_i32(Q.ctypes.data_as(P(ctypes.c_int32)), Q.shape[0], B, L, M,
+                               ctypes.byref(E), ctypes.byref(r), ctypes.byref(s))
+    assert rc == 0, rc
+    return E.value, s.value
+
+
+def run_f64(emu, Q, B, L, M):
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    E, r, s = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+    rc = emu.qbx_emu_solve_f64(Q.ctypes.data_as(P(ctypes.c_double)), Q.shape[0], B, L, M,
+                               ctypes.byref(E), ctypes.byref(r), ctypes.byref(s))
+    assert rc == 0, rc
+    return E.value, s.value
+
+
+def test_rank_helpers_match_oracle(emu):
+    for B in (0, 3, 9, 20):
+        for s in list(range(512)) + [0xFFFFF, 0x12345678]:
+            r = emu.qbx_emu_state_to_rank(s, B)
+            assert r == oracle.rank_of_state(s, B)
+            assert emu.qbx_emu_rank_to_state(r, B) == s
+
+
+@pytest.mark.parametrize("M", [1, 4, 6, 7])
+def test_every_shape_small(emu, M):
+    rng = np.random.default_rng(100 + M)
+    for trial in range(40):
+        n = int(rng.integers(M, 12))
+        Q = random_instance(rng, n, KINDS[trial % len(KINDS)])
+        B = int(rng.integers(0, n + 1)) if trial % 2 else min(n, 20)
+        ref = itertools_min(Q, B)
+        for L in range(M, n + 1):
+            assert run_i32(emu, Q, B, L, M) == ref, (n, B, L, M)
+
+
+@pytest.mark.parametrize("M", [6, 7])
+def test_medium_against_brute(emu, M):
+    rng = np.random.default_rng(7 + M)
+    for trial in range(10):
+        n = int(rng.integers(14, 21))
+        Q = random_instance(rng, n, KINDS[trial % 4])
+        B = [min(n, 20), 0, 3, n, M - 1, M + 2][trial % 6]
+        ref = oracle.brute_i64(Q, B)
+        for L in sorted({M, M + 3, M + 4, M + 5, min(n, M + 13)}):
+            if L <= n:
+                assert run_i32(emu, Q, B, L, M) == ref, (n, B, L, M)
+
+
+@pytest.mark.parametrize("M", [1, 4, 6])
+def test_float_filter_walk(emu, M):
+    rng = np.random.default_rng(31 + M)
+    for trial in range(12):
+        n = int(rng.integers(max(M, 4), 17))
+        Q = rng.standard_normal((n, n)).astype(np.float32).astype(np.float64)
+        if trial % 3 == 0:   # exact ties: integer-valued float matrix
+            Q = random_instance(rng, n, "maxcut").astype(np.float64)
+        B = [min(n, 20), 0, n // 2][trial % 3]
+        E, s = oracle.brute_f64(Q, B)
+        for L in sorted({M, min(n, M + 4), min(n, M + 9)}):
+            E2, s2 = run_f64(emu, Q, B, L, M)
+            assert s2 == s and E2 == pytest.approx(E, abs=1e-9), (n, B, L, M)
