@@ -181,12 +181,37 @@ def batch(args, ctx, devinfo):
     }
 
 
+def copy_calibration(args):
+    """A plain device-to-device copy of known size (hipMemcpy -> the runtime's copy
+    kernel), used to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    nbytes = args.copy_mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    wall = (time.perf_counter() - t0) / args.steps
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"metric": "device copy bandwidth (calibration)", "value": 2 * nbytes / wall / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": wall * 1e3, "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"hipMemcpy D2D {args.copy_mib} MiB", "bytes_read": nbytes, "bytes_written": nbytes}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["exhaustive", "float", "batch"], default="exhaustive")
+    ap.add_argument("--workload", choices=["exhaustive", "float", "batch", "copy"], default="exhaustive")
+    ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--walk-bits", type=int, default=None)
     ap.add_argument("--inner-bits", type=int, default=None)
@@ -198,6 +223,9 @@ def main():
               "(the prefix space would shard across GPUs with one 8-byte min-reduction; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        print(json.dumps(copy_calibration(args)), flush=True)
+        return
     from qubo_lib_amd import _native
     ctx = _native.get_context(0)
     devinfo = ctx.device_info()

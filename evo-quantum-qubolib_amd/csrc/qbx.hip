@@ -53,15 +53,17 @@ struct InstDev {
     int32_t n, B, L, M;
     int32_t kind;
     int32_t status;          // written by qbx_prep: 0 or QBX_ERANGE
-    uint32_t tiles;
+    uint32_t tiles;          // 64-lane tiles (lane ranks tile*64 .. tile*64+63)
+    uint32_t tpc;            // tiles per chunk: one ticket claims one chunk
+    uint32_t chunks;         // ceil(tiles / tpc); reduction slots are per chunk
     uint32_t pad0;
     const void* Q;           // source Q: int32 or double, row-major, variable order
     void* W;                 // n x LDW, bit order
     void* d;                 // n
     void* delta;             // int32[2^M]  |  double[2^M] followed by float[2^M]
     RedDev* red;
-    uint32_t* cand;          // tiles entries
-    SlotDev* slots;          // tiles entries
+    uint32_t* cand;          // chunks entries
+    SlotDev* slots;          // chunks entries
     ResultDev* result;
     double margin;           // FP32 filter margin (float kind)
     int64_t bound;           // sum |d| + sum_{a<b} |W|  (int kind)
@@ -70,8 +72,8 @@ struct InstDev {
 
 struct TileRange {
     uint32_t inst;
-    uint32_t t_begin;
-    uint32_t t_count;
+    uint32_t c_begin;        // first chunk of the range
+    uint32_t c_count;
     uint32_t pad;
     uint64_t prefix;         // first ticket of this range
 };
@@ -171,6 +173,15 @@ __device__ __forceinline__ uint64_t key_of_bits(uint64_t Ebits, int kind) {
     return (Ebits & 0x8000000000000000ull) ? ~Ebits : (Ebits | 0x8000000000000000ull);
 }
 
+// inverse of key_of_f64 (+inf for the empty key)
+__device__ __forceinline__ double energy_of_key(uint64_t k) {
+    if (k == ~0ull) return __builtin_inf();
+    const uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    union { uint64_t u; double d; } v;
+    v.u = u;
+    return v.d;
+}
+
 __device__ void finalize_problem(const InstDev& I, int lane) {
     __threadfence();
     const uint32_t cc = ld_relaxed(&I.red->cand_count);
@@ -201,17 +212,18 @@ __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uin
                                             uint64_t Ebits, uint64_t rank, int lane) {
     int last = 0;
     if (lane == 0) {
-        I.slots[tile].Ebits = Ebits;
-        I.slots[tile].rank = rank;
-        __threadfence();
         const unsigned long long old = atomicMin(&I.red->minkey, (unsigned long long)key);
         if (key <= old) {
+            // only candidate tiles are ever read back, so only they write a result slot
+            I.slots[tile].Ebits = Ebits;
+            I.slots[tile].rank = rank;
+            __threadfence();
             const uint32_t ci = atomicAdd(&I.red->cand_count, 1u);
             I.cand[ci] = tile;
         }
         __threadfence();
         const uint32_t done = atomicAdd(&I.red->done, 1u);
-        last = (done == I.tiles - 1u) ? 1 : 0;
+        last = (done == I.chunks - 1u) ? 1 : 0;
     }
     last = __shfl(last, 0);
     if (last) finalize_problem(I, lane);
@@ -219,7 +231,7 @@ __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uin
 
 __device__ __forceinline__ bool next_tile(const TileRange* __restrict__ ranges, int nranges,
                                           unsigned long long total, unsigned long long* ticket,
-                                          int lane, uint32_t& inst, uint32_t& tile) {
+                                          int lane, uint32_t& inst, uint32_t& chunk) {
     unsigned long long t = 0;
     if (lane == 0) t = atomicAdd(ticket, 1ull);
     // readfirstlane: the ticket (and so the problem, its tables and every loop bound derived
@@ -234,45 +246,63 @@ __device__ __forceinline__ bool next_tile(const TileRange* __restrict__ ranges, 
         if (ranges[mid].prefix <= t) lo = mid; else hi = mid - 1;
     }
     inst = ranges[lo].inst;
-    tile = ranges[lo].t_begin + (uint32_t)(t - ranges[lo].prefix);
+    chunk = ranges[lo].c_begin + (uint32_t)(t - ranges[lo].prefix);
     return true;
 }
 
 // ----------------------------------------------------------------------- int32 walk
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
+}
+
 template <int M, int OMAX>
 __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
                                                    unsigned long long total,
                                                    unsigned long long* ticket, int zero) {
     const int lane = threadIdx.x;
-    uint32_t inst, tile;
-    while (next_tile(ranges, nranges, total, ticket, lane, inst, tile)) {
+    uint32_t inst, chunk;
+    while (next_tile(ranges, nranges, total, ticket, lane, inst, chunk)) {
         const InstDev& I = insts[inst];
-        int32_t E = 0x7fffffff;
-        uint64_t rank = ~0ull;
-        if (I.status == 0) {
-            const int n = I.n, L = I.L, H = n - L;
-            const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
-            const bool active = (H >= 6) || (lam < (1ull << H));
-            // inner-step constants: uniform values kept in VGPRs (the lane-dependent
-            // zero offset keeps them out of the scalar register file)
-            int32_t dl[1 << M];
-            const int32_t* dsrc = (const int32_t*)I.delta;
+        // each lane's best block over the chunk: (E, lane rank, block key)
+        int32_t Eb = 0x7fffffff;
+        uint64_t lamb = ~0ull;
+        uint32_t Kb = 0xffffffffu;
+        const int n = I.n, L = I.L, B = I.B, H = n - L;
+        const int32_t* W = (const int32_t*)I.W;
+        const int32_t* d = (const int32_t*)I.d;
+        // inner-step constants: uniform values kept in VGPRs (the lane-dependent zero
+        // offset keeps them out of the scalar register file)
+        int32_t dl[1 << M];
+        const int32_t* dsrc = (const int32_t*)I.delta;
 #pragma unroll
-            for (int k = 0; k < (1 << M); ++k) dl[k] = dsrc[k + (lane & zero)];
-            if (active) {
-                const LaneOutI32 o = walk_lane_i32<M, OMAX>((const int32_t*)I.W, (const int32_t*)I.d,
-                                                            n, L, I.B, lam, dl);
-                E = o.E;
-                rank = o.rank;
+        for (int k = 0; k < (1 << M); ++k) dl[k] = dsrc[k + (lane & zero)];
+        if (I.status == 0) {
+            const uint32_t t0 = chunk * I.tpc;
+            const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
+            for (uint32_t tile = t0; tile < t1; ++tile) {
+                const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
+                if (H >= 6 || lam < (1ull << H)) {
+                    const int64_t key = lane_best_block_i32<M, OMAX>(W, d, n, L, B, lam, dl);
+                    if (key_E(key) < Eb) { Eb = key_E(key); lamb = lam; Kb = key_K(key); }
+                }
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
-            const int32_t E2 = __shfl_xor(E, off);
-            const uint64_t r2 = __shfl_xor(rank, off);
-            if (E2 < E || (E2 == E && r2 < rank)) { E = E2; rank = r2; }
+            const int32_t E2 = __shfl_xor(Eb, off);
+            const uint64_t l2 = __shfl_xor(lamb, off);
+            const uint32_t K2 = __shfl_xor(Kb, off);
+            if (E2 < Eb || (E2 == Eb && (l2 < lamb || (l2 == lamb && K2 < Kb)))) { Eb = E2; lamb = l2; Kb = K2; }
         }
-        commit_tile(I, tile, key_of_i64(E), (uint64_t)(int64_t)E, rank, lane);
+        // the wave's winning block is now uniform: re-walk it once for its lowest inner rank
+        Eb = (int32_t)uni32((uint32_t)Eb);
+        lamb = uni64(lamb);
+        Kb = uni32(Kb);
+        uint64_t rank = ~0ull;
+        if (Eb != 0x7fffffff)
+            rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dl);
+        commit_tile(I, chunk, key_of_i64(Eb), (uint64_t)(int64_t)Eb, rank, lane);
     }
 }
 
@@ -283,25 +313,37 @@ __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restrict__ i
                                                    unsigned long long total,
                                                    unsigned long long* ticket, int zero) {
     const int lane = threadIdx.x;
-    uint32_t inst, tile;
-    while (next_tile(ranges, nranges, total, ticket, lane, inst, tile)) {
+    uint32_t inst, chunk;
+    while (next_tile(ranges, nranges, total, ticket, lane, inst, chunk)) {
         const InstDev& I = insts[inst];
         double E = __builtin_inf();
         uint64_t rank = ~0ull;
         if (I.status == 0) {
             const int n = I.n, L = I.L, H = n - L;
-            const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
-            const bool active = (H >= 6) || (lam < (1ull << H));
             const double* dld = (const double*)I.delta;
             const float* dsrc = (const float*)(dld + (1 << M));
             float dls[1 << M];
 #pragma unroll
             for (int k = 0; k < (1 << M); ++k) dls[k] = dsrc[k + (lane & zero)];
-            if (active) {
-                const LaneOutF64 o = walk_lane_f64<M, OMAX>((const double*)I.W, (const double*)I.d,
-                                                            n, L, I.B, lam, dls, dld, I.margin);
-                E = o.E;
-                rank = o.rank;
+            const uint32_t t0 = chunk * I.tpc;
+            const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
+            for (uint32_t tile = t0; tile < t1; ++tile) {
+                // branch and bound across waves: the problem's running minimum key holds an
+                // exact energy of some state, so it bounds the answer from above
+                const double bound = energy_of_key(ld_relaxed((const uint64_t*)&I.red->minkey));
+                const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
+                if (H >= 6 || lam < (1ull << H)) {
+                    const LaneOutF64 o = walk_lane_f64<M, OMAX>((const double*)I.W, (const double*)I.d,
+                                                                n, L, I.B, lam, dls, dld, I.margin, bound);
+                    if (o.E < E || (o.E == E && o.rank < rank)) { E = o.E; rank = o.rank; }
+                }
+                // publish the wave's best so far (an exact energy) for the other waves
+                double Ew = E;
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double e2 = __shfl_xor(Ew, off);
+                    Ew = e2 < Ew ? e2 : Ew;
+                }
+                if (lane == 0 && Ew < bound) atomicMin(&I.red->minkey, (unsigned long long)key_of_f64(Ew));
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
@@ -311,7 +353,7 @@ __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restrict__ i
         }
         union { double d; uint64_t u; } cv;
         cv.d = E;
-        commit_tile(I, tile, key_of_f64(E), cv.u, rank, lane);
+        commit_tile(I, chunk, key_of_f64(E), cv.u, rank, lane);
     }
 }
 
@@ -416,13 +458,14 @@ int pick_variant(int kind, int want_M, int n) {
     return best;
 }
 
-// Default walk bits per lane.  `single`: a lone problem should be cut into >= 2^14 tiles
-// where possible (fills 256 CUs with a short tail); inside a batch fewer, larger tiles
-// keep per-tile set-up small while many problems share the GPU.
+// Default walk bits per lane (measured on MI355X, int32 kernel): a lone problem is cut
+// into about 2^10 .. 2^22 tiles -- L = n - 16, capped at 18 (n = 28: 12, 32: 16, 36 and
+// 40: 18); inside a batch many problems share the GPU, so fewer, larger tiles per problem
+// keep the per-lane set-up small.
 int default_walk_bits(int n, int M, int OMAX, bool single) {
-    const int spare = single ? 20 : 10;
-    int L = n - spare;
-    if (L > 16) L = 16;
+    int L = single ? n - 16 : n - 10;
+    const int cap = single ? 18 : 16;
+    if (L > cap) L = cap;
     if (L < M) L = M;
     if (L < n - MAX_LANE_BITS) L = n - MAX_LANE_BITS;
     if (L > M + OMAX) L = M + OMAX;
@@ -432,7 +475,7 @@ int default_walk_bits(int n, int M, int OMAX, bool single) {
 
 struct Plan {
     int n, B, L, M, kind, variant, H;
-    uint32_t tiles;
+    uint32_t tiles, tpc, chunks;
     size_t off_Q, off_W, off_d, off_delta, off_cand, off_slots;
 };
 
@@ -477,6 +520,18 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         p.L = L;
         p.H = n - L;
         p.tiles = p.H > 6 ? (1u << (p.H - 6)) : 1u;
+        // tiles per chunk: amortise the per-ticket work (atomics, table loads, the final
+        // re-walk) over about 2^22 states, but keep enough chunks to spread one problem
+        // over the GPU (single solves) or over a few waves (inside a batch)
+        {
+            uint32_t tpc = (22 - 6 - L) > 0 ? (1u << (22 - 6 - L)) : 1u;
+            const uint32_t min_chunks = single ? 8192u : 16u;
+            const uint32_t cap = p.tiles / min_chunks;
+            if (tpc > cap) tpc = cap;
+            if (tpc < 1u) tpc = 1u;
+            p.tpc = tpc;
+            p.chunks = (p.tiles + tpc - 1) / tpc;
+        }
         p.off_Q = off;
         off = align256(off + esize * (size_t)n * n);
     }
@@ -494,11 +549,11 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     off = align256(off + sizeof(RedDev) * count);
     for (int i = 0; i < count; ++i) {
         plans[i].off_cand = off;
-        off = align256(off + 4 * (size_t)plans[i].tiles);
+        off = align256(off + 4 * (size_t)plans[i].chunks);
     }
     for (int i = 0; i < count; ++i) {
         plans[i].off_slots = off;
-        off = align256(off + sizeof(SlotDev) * (size_t)plans[i].tiles);
+        off = align256(off + sizeof(SlotDev) * (size_t)plans[i].chunks);
     }
     const size_t off_res = off;
     off = align256(off + sizeof(ResultDev) * count);
@@ -516,7 +571,10 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     for (int i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         if (plans[a].variant != plans[b].variant) return plans[a].variant < plans[b].variant;
-        return plans[a].L > plans[b].L;
+        // longest chunks first: chunk work = tpc * 64 * 2^L states
+        const int wa = plans[a].L + 31 - __builtin_clz(plans[a].tpc);
+        const int wb = plans[b].L + 31 - __builtin_clz(plans[b].tpc);
+        return wa > wb;
     });
 
     InstDev* hinst = (InstDev*)(h + off_inst);
@@ -526,6 +584,7 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         InstDev& I = hinst[i];
         memset(&I, 0, sizeof I);
         I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = kind; I.status = 0; I.tiles = p.tiles;
+        I.tpc = p.tpc; I.chunks = p.chunks;
         I.Q = dv + p.off_Q;
         I.W = dv + p.off_W;
         I.d = dv + p.off_d;
@@ -547,8 +606,8 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
             prefix = 0;
         }
         TileRange& R = hr[r];
-        R.inst = (uint32_t)i; R.t_begin = 0; R.t_count = plans[i].tiles; R.pad = 0; R.prefix = prefix;
-        prefix += plans[i].tiles;
+        R.inst = (uint32_t)i; R.c_begin = 0; R.c_count = plans[i].chunks; R.pad = 0; R.prefix = prefix;
+        prefix += plans[i].chunks;
         groups.back().cnt++;
         groups.back().total = prefix;
     }

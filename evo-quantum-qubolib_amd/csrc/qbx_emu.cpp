@@ -25,14 +25,21 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
     for (uint32_t t = 0; t < (1u << M); ++t) dl[t] = delta_elem<int32_t, int32_t>(Q, n, M, t);
     const int H = n - L;
     const uint64_t lanes = 1ull << H;
+    // the device flow: every lane's best block key, the (E, lane rank, block key) winner
+    // over all lanes, then one re-walk of the winning block
     int32_t bE = 0x7fffffff;
-    uint64_t bR = ~0ull;
+    uint64_t bLam = ~0ull;
+    uint32_t bK = 0xffffffffu;
     for (uint64_t lam = 0; lam < lanes; ++lam) {
-        LaneOutI32 o = walk_lane_i32<M, OMAX>(W.data(), d.data(), n, L, B, lam, dl.data());
-        if (o.E < bE || (o.E == bE && o.rank < bR)) { bE = o.E; bR = o.rank; }
+        const int64_t key = lane_best_block_i32<M, OMAX>(W.data(), d.data(), n, L, B, lam, dl.data());
+        if (key_E(key) < bE) { bE = key_E(key); bLam = lam; bK = key_K(key); }
     }
     *E_out = bE;
-    *rank_out = bR;
+    *rank_out = block_rank_hi(bLam, L, M, bK) |
+                block_rank_lo_i32<M>(W.data(), d.data(), n, L, B, bLam, bE, bK, dl.data());
+    // the per-lane composition must agree with it
+    const LaneOutI32 o = walk_lane_i32<M, OMAX>(W.data(), d.data(), n, L, B, bLam, dl.data());
+    if (o.E != bE || o.rank != *rank_out) return -3;
     return 0;
 }
 
@@ -59,8 +66,9 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
     const uint64_t lanes = 1ull << H;
     double bE = __builtin_inf();
     uint64_t bR = ~0ull;
+    // lanes see the best energy found so far as their bound, as the device's waves do
     for (uint64_t lam = 0; lam < lanes; ++lam) {
-        LaneOutF64 o = walk_lane_f64<M, OMAX>(W.data(), d.data(), n, L, B, lam, dls.data(), dld.data(), margin);
+        LaneOutF64 o = walk_lane_f64<M, OMAX>(W.data(), d.data(), n, L, B, lam, dls.data(), dld.data(), margin, bE);
         if (o.E < bE || (o.E == bE && o.rank < bR)) { bE = o.E; bR = o.rank; }
     }
     *E_out = bE;

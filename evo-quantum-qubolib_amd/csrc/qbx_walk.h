@@ -304,47 +304,92 @@ QBX_HD void group_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const 
     (void)(group_step_i32<M, OMAX, Js>(s, W, dl, c0, nblk, SBk, pmaskS, best) && ...);
 }
 
-// Walk one lane.  W: bit-ordered, LDW columns; d: n; dl: delta_t for t < 2^M.
-// Returns the lexicographic minimum (E, rank) over the lane's 2^L states.
+// Lane set-up: E0 and the fields of the lane's first block.  The lane prefix p splits into
+// its low 6 bits (per lane) and the bits above them, which depend only on lam >> 6 and so
+// are the same for the 64 lanes of a wave: their contributions are accumulated once, from
+// wave-uniform values (scalar registers on the device), and only the low bits cost
+// per-lane work.
 template <int M, int OMAX>
-QBX_HD LaneOutI32 walk_lane_i32(const int32_t* __restrict__ W, const int32_t* __restrict__ d,
-                                int n, int L, int B, uint64_t lam, const int32_t* dl) {
-    static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
+QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* __restrict__ d,
+                           int n, int L, int B, uint64_t lam) {
     const int H = n - L;
     const int O = L - M;
+    const int Hlo = H < 6 ? H : 6;
     const uint64_t p = lane_prefix(lam, L, B);
+    const uint64_t p_hi = shr64(lane_prefix(lam & ~63ull, L, B), 6);   // wave-uniform
+    const uint32_t p_lo = (uint32_t)(p & 63ull);
 
-    LaneI32<M, OMAX> s;
-    uint32_t E0 = 0;
+    // contributions of the set high lane bits (uniform)
+    uint32_t Ec = 0;
+    int32_t fc[M], uc[OMAX], lc[6];
 #pragma unroll
-    for (int k = 0; k < M; ++k) s.f[k] = d[k];
+    for (int k = 0; k < M; ++k) fc[k] = d[k];
 #pragma unroll
-    for (int q = 0; q < OMAX; ++q) s.u[q] = (q < O) ? d[M + q] : 0;
-    for (int j = 0; j < H; ++j) {
+    for (int q = 0; q < OMAX; ++q) uc[q] = (q < O) ? d[M + q] : 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) lc[i] = (i < Hlo) ? d[L + i] : 0;
+    for (int j = Hlo; j < H; ++j) {
+        if (!((p_hi >> (j - Hlo)) & 1ull)) continue;
         const int a = L + j;
-        const uint32_t bj = (uint32_t)((p >> j) & 1ull);
-        const uint32_t mj = 0u - bj;
         const int32_t* Wa = W + a * LDW;
         uint32_t e = (uint32_t)d[a];
         for (int j2 = j + 1; j2 < H; ++j2)
-            e += (uint32_t)Wa[L + j2] & (0u - (uint32_t)((p >> j2) & 1ull));
-        E0 += e & mj;
+            if ((p_hi >> (j2 - Hlo)) & 1ull) e += (uint32_t)Wa[L + j2];
+        Ec += e;
 #pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] += (int32_t)((uint32_t)Wa[k] & mj);
+        for (int k = 0; k < M; ++k) fc[k] += Wa[k];
 #pragma unroll
-        for (int q = 0; q < OMAX; ++q) s.u[q] += (int32_t)((uint32_t)Wa[M + q] & mj);
+        for (int q = 0; q < OMAX; ++q) uc[q] += Wa[M + q];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) lc[i] += (i < Hlo) ? Wa[L + i] : 0;
+    }
+    // per-lane part: the low lane bits (lc[i] already holds d + the high-bit couplings)
+    uint32_t E0 = Ec;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] = fc[k];
+#pragma unroll
+    for (int q = 0; q < OMAX; ++q) s.u[q] = uc[q];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        if (i >= Hlo) break;
+        const uint32_t mi = 0u - ((p_lo >> i) & 1u);
+        const int a = L + i;
+        const int32_t* Wa = W + a * LDW;
+        uint32_t e = (uint32_t)lc[i];
+#pragma unroll
+        for (int i2 = i + 1; i2 < 6; ++i2)
+            if (i2 < Hlo) e += (uint32_t)Wa[L + i2] & (0u - ((p_lo >> i2) & 1u));
+        E0 += e & mi;
+#pragma unroll
+        for (int k = 0; k < M; ++k) s.f[k] += (int32_t)((uint32_t)Wa[k] & mi);
+#pragma unroll
+        for (int q = 0; q < OMAX; ++q) s.u[q] += (int32_t)((uint32_t)Wa[M + q] & mi);
     }
     s.E0 = (int32_t)E0;
 #pragma unroll
     for (int k = 0; k < M; ++k) s.nf[k] = -s.f[k];
+}
+
+// Lane key: (E, K) packed as one signed 64-bit value, E in the high word.  Ordering lane
+// keys orders the lane's blocks by (block minimum, block rank).
+QBX_HD int32_t key_E(int64_t key) { return (int32_t)(key >> 32); }
+QBX_HD uint32_t key_K(int64_t key) { return (uint32_t)key; }
+constexpr int64_t KEY_NONE = 0x7fffffffffffffffll;
+
+// Walk one lane's 2^L states and return the key of its best block.
+// W: bit-ordered, LDW columns; d: n; dl: delta_t for t < 2^M.
+template <int M, int OMAX>
+QBX_HD int64_t lane_best_block_i32(const int32_t* __restrict__ W, const int32_t* __restrict__ d,
+                                   int n, int L, int B, uint64_t lam, const int32_t* dl) {
+    static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
+    const int O = L - M;
+    const uint64_t p = lane_prefix(lam, L, B);
+    LaneI32<M, OMAX> s;
+    lane_setup_i32<M, OMAX>(s, W, d, n, L, B, lam);
 
     const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
-    const int parp = parity64(p);
-    const uint32_t pmaskS = (parp && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
-
-    // best block so far as one signed 64-bit key: (block minimum, block key K); the block
-    // index is recovered from K at the end.
-    int64_t best = 0x7fffffffffffffffll;
+    const uint32_t pmaskS = (parity64(p) && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
+    int64_t best = KEY_NONE;
     const uint32_t nblk = 1u << O;
     for (uint32_t c0 = 0;;) {
         group_i32<M, OMAX>(s, W, dl, c0, nblk, SBk, pmaskS, best,
@@ -355,32 +400,48 @@ QBX_HD LaneOutI32 walk_lane_i32(const int32_t* __restrict__ W, const int32_t* __
         const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
         trans_dyn_i32<M, OMAX>(s, W + (M + i) * LDW, i, up);
     }
+    return best;
+}
 
-    // recover the best block's index c from its key, rebuild its state, re-walk it once
-    const int32_t bestE = (int32_t)(best >> 32);
-    const uint32_t bestK = (uint32_t)best;
-    uint32_t c;
-    {
-        const uint32_t k = bestK ^ pmaskS;
-        // k = ((c >> SBk) << SBk) | (gray(c) & lowmask(SBk)); gray(c) = c ^ (c >> 1)
-        c = (k >> SBk) << SBk;
-        for (int b = SBk - 1; b >= 0; --b) {
-            const uint32_t cb = ((k >> b) & 1u) ^ ((c >> (b + 1)) & 1u);
-            c |= cb << b;
-        }
-    }
-    const uint64_t hs = (p << O) | (uint64_t)(c ^ (c >> 1));
+// Lowest inner rank of the minimisers inside block K of lane lam (whose minimum is E):
+// rebuild the block's state from scratch and re-walk it once.
+template <int M>
+QBX_HD uint32_t block_rank_lo_i32(const int32_t* __restrict__ W, const int32_t* __restrict__ d,
+                                  int n, int L, int B, uint64_t lam, int32_t E, uint32_t K,
+                                  const int32_t* dl) {
+    const int O = L - M;
+    const uint64_t p = lane_prefix(lam, L, B);
+    const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
+    const int parp = parity64(p);
+    const uint32_t pmaskS = (parp && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
+    // invert K = ((c >> SBk) << SBk) | (gray(c) & lowmask(SBk)), then XOR pmaskS
+    const uint32_t k = K ^ pmaskS;
+    uint32_t c = (k >> SBk) << SBk;
+    for (int b = SBk - 1; b >= 0; --b) c |= (((k >> b) & 1u) ^ ((c >> (b + 1)) & 1u)) << b;
+    const uint32_t gc = c ^ (c >> 1);
+    const uint64_t hs = (p << O) | (uint64_t)gc;
     int32_t bE0;
     int32_t bf[M], bnf[M];
     block_state_i32<M>(W, d, n, hs, bE0, bf);
 #pragma unroll
-    for (int k = 0; k < M; ++k) bnf[k] = -bf[k];
-    const int par = parp ^ parity32(c ^ (c >> 1));
-    const uint32_t bestLo = block_argmin_i32<M>(bE0, bf, bnf, dl, bestE, B, par);
+    for (int kk = 0; kk < M; ++kk) bnf[kk] = -bf[kk];
+    return block_argmin_i32<M>(bE0, bf, bnf, dl, E, B, parp ^ parity32(gc));
+}
 
+// Rank of the best state of block K of lane lam.
+QBX_HD uint64_t block_rank_hi(uint64_t lam, int L, int M, uint32_t K) {
+    return shl64(lam, L) | ((uint64_t)K << M);
+}
+
+// Whole-lane answer (used by the host build): best block, then its re-walk.
+template <int M, int OMAX>
+QBX_HD LaneOutI32 walk_lane_i32(const int32_t* __restrict__ W, const int32_t* __restrict__ d,
+                                int n, int L, int B, uint64_t lam, const int32_t* dl) {
+    const int64_t key = lane_best_block_i32<M, OMAX>(W, d, n, L, B, lam, dl);
     LaneOutI32 out;
-    out.E = bestE;
-    out.rank = shl64(lam, L) | ((uint64_t)bestK << M) | (uint64_t)bestLo;
+    out.E = key_E(key);
+    out.rank = block_rank_hi(lam, L, M, key_K(key)) |
+               block_rank_lo_i32<M>(W, d, n, L, B, lam, key_E(key), key_K(key), dl);
     return out;
 }
 
@@ -455,7 +516,7 @@ QBX_HD void trans_dyn_f64(LaneF64<M, OMAX>& s, const double* __restrict__ WJ, in
     }
 }
 
-// next float above r (r finite); +inf and NaN are returned unchanged.
+// f32 value >= x (x finite or +inf): x rounded to float, bumped up one ulp if it fell below.
 QBX_HD float f32_next_up(float r) {
     union { float f; uint32_t u; } v;
     v.f = r;
@@ -464,37 +525,59 @@ QBX_HD float f32_next_up(float r) {
     if (v.u & 0x80000000u) v.u -= 1u; else v.u += 1u;
     return v.f;
 }
-// f32 value >= x (x finite or +inf): x rounded to float, bumped up one ulp if it fell below.
 QBX_HD float f32_at_least(double x) {
     float r = (float)x;
     if ((double)r < x) r = f32_next_up(r);
     return r;
 }
 
-// dld: FP64 inner deltas (uniform, read only when a block is re-walked);
-// dls: FP32 copies held in registers.
+// Lane set-up in FP64 (same split as lane_setup_i32: high lane bits are wave-uniform).
 template <int M, int OMAX>
-QBX_HD LaneOutF64 walk_lane_f64(const double* __restrict__ W, const double* __restrict__ d,
-                                int n, int L, int B, uint64_t lam, const float* dls,
-                                const double* __restrict__ dld, double margin) {
-    static_assert(OMAX >= GROUP_BITS, "OMAX must cover the static transition cases");
+QBX_HD void lane_setup_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const double* __restrict__ d,
+                           int n, int L, int B, uint64_t lam) {
     const int H = n - L;
     const int O = L - M;
+    const int Hlo = H < 6 ? H : 6;
     const uint64_t p = lane_prefix(lam, L, B);
-
-    LaneF64<M, OMAX> s;
-    double E0 = 0.0;
+    const uint64_t p_hi = shr64(lane_prefix(lam & ~63ull, L, B), 6);
+    const uint32_t p_lo = (uint32_t)(p & 63ull);
+    double Ec = 0.0, fc[M], uc[OMAX], lc[6];
 #pragma unroll
-    for (int k = 0; k < M; ++k) s.f[k] = d[k];
+    for (int k = 0; k < M; ++k) fc[k] = d[k];
 #pragma unroll
-    for (int q = 0; q < OMAX; ++q) s.u[q] = (q < O) ? d[M + q] : 0.0;
-    for (int j = 0; j < H; ++j) {
-        if (!((p >> j) & 1ull)) continue;
+    for (int q = 0; q < OMAX; ++q) uc[q] = (q < O) ? d[M + q] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) lc[i] = (i < Hlo) ? d[L + i] : 0.0;
+    for (int j = Hlo; j < H; ++j) {
+        if (!((p_hi >> (j - Hlo)) & 1ull)) continue;
         const int a = L + j;
         const double* Wa = W + a * LDW;
         double e = d[a];
         for (int j2 = j + 1; j2 < H; ++j2)
-            if ((p >> j2) & 1ull) e += Wa[L + j2];
+            if ((p_hi >> (j2 - Hlo)) & 1ull) e += Wa[L + j2];
+        Ec += e;
+#pragma unroll
+        for (int k = 0; k < M; ++k) fc[k] += Wa[k];
+#pragma unroll
+        for (int q = 0; q < OMAX; ++q) uc[q] += Wa[M + q];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) lc[i] += (i < Hlo) ? Wa[L + i] : 0.0;
+    }
+    double E0 = Ec;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] = fc[k];
+#pragma unroll
+    for (int q = 0; q < OMAX; ++q) s.u[q] = uc[q];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        if (i >= Hlo) break;
+        if (!((p_lo >> i) & 1u)) continue;
+        const int a = L + i;
+        const double* Wa = W + a * LDW;
+        double e = lc[i];
+#pragma unroll
+        for (int i2 = i + 1; i2 < 6; ++i2)
+            if (i2 < Hlo && ((p_lo >> i2) & 1u)) e += Wa[L + i2];
         E0 += e;
 #pragma unroll
         for (int k = 0; k < M; ++k) s.f[k] += Wa[k];
@@ -502,58 +585,126 @@ QBX_HD LaneOutF64 walk_lane_f64(const double* __restrict__ W, const double* __re
         for (int q = 0; q < OMAX; ++q) s.u[q] += Wa[M + q];
     }
     s.E0 = E0;
+}
+
+// E0 and inner fields of an arbitrary block from scratch (hs: bits above the block).
+template <int M>
+QBX_HD void block_state_f64(const double* __restrict__ W, const double* __restrict__ d, int n,
+                            uint64_t hs, double& E0, double* f) {
+    double E = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) f[k] = d[k];
+    for (int a = M; a < n; ++a) {
+        if (!((hs >> (a - M)) & 1ull)) continue;
+        const double* Wa = W + a * LDW;
+        E += d[a];
+        for (int b = a + 1; b < n; ++b)
+            if ((hs >> (b - M)) & 1ull) E += Wa[b];
+#pragma unroll
+        for (int k = 0; k < M; ++k) f[k] += Wa[k];
+    }
+    E0 = E;
+}
+
+// Exact FP64 walk of one block; folds every state into the lane's (bestE, bestR).
+template <int M>
+QBX_HD void block_rewalk_f64(double E0, const double* f, const double* __restrict__ dld,
+                             uint64_t rblk, int B, int par, double& bestE, uint64_t& bestR) {
+    double E = E0;
+    {
+        const uint64_t r = rblk | rank_lo<M>(0u, B, par);
+        if (E < bestE || (E == bestE && r < bestR)) { bestE = E; bestR = r; }
+    }
+    for (int t = 1; t < (1 << M); ++t) {
+        const int k = __builtin_ctz((unsigned)t);
+        const uint32_t g = (uint32_t)t ^ ((uint32_t)t >> 1);
+        const bool up = ((g >> k) & 1u) != 0u;
+        double fk = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < M; ++kk) if (kk == k) fk = f[kk];
+        E = up ? (E + fk) : (E - fk);
+        E = E + dld[t];
+        if (E <= bestE) {
+            const uint64_t r = rblk | rank_lo<M>(g, B, par);
+            if (E < bestE || r < bestR) { bestE = E; bestR = r; }
+        }
+    }
+}
+
+// One block of an unrolled float group: static transition, FP32 block minimum from the FP64
+// slow state, candidate bit when the minimum is within the lane's threshold.
+template <int M, int OMAX, int J>
+QBX_HD bool group_step_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const float* dls,
+                           uint32_t c0, uint32_t nblk, float thr, uint32_t& cand) {
+    const uint32_t c = c0 + (uint32_t)J;
+    if (J > 0) {
+        if (c >= nblk) return false;
+        constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
+        const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
+                                             : (((c0 >> GROUP_BITS) & 1u) == 0u);
+        trans_static_f64<M, OMAX, i>(s, W + (M + i) * LDW, up);
+    }
+    float fs[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) fs[k] = (float)s.f[k];
+    const float mn = block_min_f32<M>((float)s.E0, fs, dls);
+    if (mn <= thr) cand |= 1u << J;
+    return true;
+}
+
+template <int M, int OMAX, int... Js>
+QBX_HD void group_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const float* dls,
+                      uint32_t c0, uint32_t nblk, float thr, uint32_t& cand,
+                      std::integer_sequence<int, Js...>) {
+    (void)(group_step_f64<M, OMAX, Js>(s, W, dls, c0, nblk, thr, cand) && ...);
+}
+
+// dls: FP32 inner deltas (registers); dld: FP64 inner deltas (memory, read on re-walks).
+// `bound`: an exact energy already reached by some state of the problem (+inf if none):
+// blocks whose FP32 minimum exceeds min(lane best, bound) + margin cannot hold the answer.
+template <int M, int OMAX>
+QBX_HD LaneOutF64 walk_lane_f64(const double* __restrict__ W, const double* __restrict__ d,
+                                int n, int L, int B, uint64_t lam, const float* dls,
+                                const double* __restrict__ dld, double margin, double bound) {
+    static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
+    const int O = L - M;
+    const uint64_t p = lane_prefix(lam, L, B);
+    LaneF64<M, OMAX> s;
+    lane_setup_f64<M, OMAX>(s, W, d, n, L, B, lam);
 
     const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
     const int parp = parity64(p);
     const uint32_t pmaskS = (parp && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
+    const uint64_t laneRank = shl64(lam, L);
 
+    // the lane's first block is walked exactly at once, so the filter starts tight
     double bestE = __builtin_inf();
     uint64_t bestR = ~0ull;
-    float thr = __builtin_inff();
-    const uint64_t laneRank = shl64(lam, L);
+    block_rewalk_f64<M>(s.E0, s.f, dld, laneRank | ((uint64_t)block_key(0u, SBk, pmaskS) << M), B, parp,
+                        bestE, bestR);
+    float thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
+
     const uint32_t nblk = 1u << O;
-    uint32_t c = 0;
-    for (;;) {
-        float fs[M];
-#pragma unroll
-        for (int k = 0; k < M; ++k) fs[k] = (float)s.f[k];
-        const float mn = block_min_f32<M>((float)s.E0, fs, dls);
-        if (mn <= thr) {
-            const uint32_t K = block_key(c, SBk, pmaskS);
-            const int par = parp ^ parity32(c ^ (c >> 1));
-            const uint64_t rblk = laneRank | ((uint64_t)K << M);
-            double E = s.E0;
-            {
-                const uint64_t r = rblk | rank_lo<M>(0u, B, par);
-                if (E < bestE || (E == bestE && r < bestR)) { bestE = E; bestR = r; }
-            }
-            for (int t = 1; t < (1 << M); ++t) {
-                const int k = __builtin_ctz((unsigned)t);
-                const uint32_t g = (uint32_t)t ^ ((uint32_t)t >> 1);
-                const bool up = ((g >> k) & 1u) != 0u;
-                double fk = 0.0;
-#pragma unroll
-                for (int kk = 0; kk < M; ++kk) if (kk == k) fk = s.f[kk];
-                E = up ? (E + fk) : (E - fk);
-                E = E + dld[t];
-                if (E <= bestE) {
-                    const uint64_t r = rblk | rank_lo<M>(g, B, par);
-                    if (E < bestE || r < bestR) { bestE = E; bestR = r; }
-                }
-            }
-            thr = f32_at_least(bestE + margin);
+    for (uint32_t c0 = 0;;) {
+        uint32_t cand = 0u;
+        group_f64<M, OMAX>(s, W, dls, c0, nblk, thr, cand, std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
+        if (c0 == 0) cand &= ~1u;   // block 0 is already exact
+        while (cand) {               // rare: rebuild each candidate block and walk it in FP64
+            const uint32_t j = (uint32_t)__builtin_ctz(cand);
+            cand &= cand - 1u;
+            const uint32_t c = c0 + j;
+            const uint32_t gc = c ^ (c >> 1);
+            double bE0, bf[M];
+            block_state_f64<M>(W, d, n, (p << O) | (uint64_t)gc, bE0, bf);
+            block_rewalk_f64<M>(bE0, bf, dld, laneRank | ((uint64_t)block_key(c, SBk, pmaskS) << M), B,
+                                parp ^ parity32(gc), bestE, bestR);
+            thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
         }
-        if (++c >= nblk) break;
-        const int i = ctz32(c);
-        const bool up = (((c ^ (c >> 1)) >> i) & 1u) != 0u;
-        const double* WJ = W + (M + i) * LDW;
-        switch (i) {
-            case 0: trans_static_f64<M, OMAX, 0>(s, WJ, up); break;
-            case 1: trans_static_f64<M, OMAX, 1>(s, WJ, up); break;
-            case 2: trans_static_f64<M, OMAX, 2>(s, WJ, up); break;
-            case 3: trans_static_f64<M, OMAX, 3>(s, WJ, up); break;
-            default: trans_dyn_f64<M, OMAX>(s, WJ, i, up); break;
-        }
+        c0 += 1u << GROUP_BITS;
+        if (c0 >= nblk) break;
+        const int i = ctz32(c0);
+        const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
+        trans_dyn_f64<M, OMAX>(s, W + (M + i) * LDW, i, up);
     }
     LaneOutF64 out;
     out.E = bestE;

@@ -6,8 +6,10 @@ Arithmetic path per problem:
       (Q'_ij = Q_ij + Q_ji, i < j), which has the same energies and fits int32 whenever
       the bound does.
   * integer Q with bound < 2^53: the FP64 walk, exact on integers (qbx_solve_f64).
-  * float Q: the FP64 walk on the float64 values of Q (an FP32 block filter, every
-      candidate block re-walked in FP64).
+  * float Q whose entries are all integers (MaxCut's default float32 matrices): the
+      exact int32 walk, energies returned as floats.
+  * other float Q: the FP64 walk on the float64 values of Q (an FP32 block filter,
+      every candidate block re-walked in FP64).
   * anything else: OverflowError.  There is no CPU path: without libqbx.so or a GPU
     every call raises.
 
@@ -76,6 +78,12 @@ def prepare(Q) -> Prepared:
         Qf = Q.astype(np.float64)
         if not np.all(np.isfinite(Qf)):
             raise ValueError("Q contains non-finite values")
+        if np.all(Qf == np.rint(Qf)) and np.abs(Qf).max(initial=0.0) < 2.0 ** 53:
+            # integer-valued float Q (e.g. MaxCut's default float32 matrices): the energies
+            # are integers, so the exact int32 walk gives the same answer; E stays a float
+            p = prepare(Qf.astype(np.int64))
+            if p.kind == "i32":
+                return Prepared(p.n, "i32", False, p.matrix, p.bound)
         bound = float(np.abs(np.diag(Qf)).sum() + np.abs(np.triu(Qf, 1) + np.triu(Qf.T, 1)).sum())
         return Prepared(n, "f64", False, np.ascontiguousarray(Qf), bound)
     raise TypeError(f"unsupported Q dtype {Q.dtype}")
@@ -84,6 +92,8 @@ def prepare(Q) -> Prepared:
 def _finish(p: Prepared, r: _native.Result) -> RawResult:
     if p.integer:
         E = int(r.E_int) if p.kind == "i32" else int(round(r.E))
+    elif p.kind == "i32":
+        E = float(r.E_int)
     else:
         E = float(r.E)
     info = dict(kernel_ms=r.kernel_ms, states=int(r.states), tiles=int(r.tiles), walk_bits=r.walk_bits,

@@ -21,8 +21,11 @@ def test_path_selection():
     assert engine.prepare(c).kind == "i32"
     with pytest.raises(OverflowError):
         engine.prepare(np.array([[2 ** 62, 0], [0, 2 ** 62]], dtype=np.int64))
-    f = engine.prepare(np.eye(3, dtype=np.float32))
+    f = engine.prepare(np.eye(3, dtype=np.float32) * 0.5)
     assert f.kind == "f64" and not f.integer
+    # integer-valued floats take the exact int32 walk (energies stay floats)
+    g = engine.prepare(np.eye(3, dtype=np.float32))
+    assert g.kind == "i32" and not g.integer
     with pytest.raises(ValueError):
         engine.prepare(np.array([[np.nan]]))
     with pytest.raises(ValueError):
