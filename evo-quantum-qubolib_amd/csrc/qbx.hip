@@ -307,6 +307,26 @@ __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restrict__ i
 }
 
 // ------------------------------------------------------------------ float-filter walk
+// Running upper bound shared by every wave of one problem: the problem's reduction key
+// (an exact energy of some state, or empty).  publish() takes the wave minimum of the
+// lanes' exact bests and lowers the key when it improved on what this wave last offered.
+// All 64 lanes must call it together (it shuffles across the wave).
+struct WaveBound {
+    unsigned long long* minkey;
+    double offered;
+    __device__ double refresh() const { return energy_of_key(ld_relaxed((const uint64_t*)minkey)); }
+    __device__ void publish(double e) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const double e2 = __shfl_xor(e, off);
+            e = e2 < e ? e2 : e;
+        }
+        if (e < offered) {
+            offered = e;
+            if (threadIdx.x == 0) atomicMin(minkey, (unsigned long long)key_of_f64(e));
+        }
+    }
+};
+
 template <int M, int OMAX>
 __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
@@ -327,23 +347,17 @@ __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restrict__ i
             for (int k = 0; k < (1 << M); ++k) dls[k] = dsrc[k + (lane & zero)];
             const uint32_t t0 = chunk * I.tpc;
             const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
+            // branch and bound across waves through the problem's running minimum key
+            WaveBound wb{&I.red->minkey, __builtin_inf()};
             for (uint32_t tile = t0; tile < t1; ++tile) {
-                // branch and bound across waves: the problem's running minimum key holds an
-                // exact energy of some state, so it bounds the answer from above
-                const double bound = energy_of_key(ld_relaxed((const uint64_t*)&I.red->minkey));
-                const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
-                if (H >= 6 || lam < (1ull << H)) {
-                    const LaneOutF64 o = walk_lane_f64<M, OMAX>((const double*)I.W, (const double*)I.d,
-                                                                n, L, I.B, lam, dls, dld, I.margin, bound);
-                    if (o.E < E || (o.E == E && o.rank < rank)) { E = o.E; rank = o.rank; }
-                }
-                // publish the wave's best so far (an exact energy) for the other waves
-                double Ew = E;
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double e2 = __shfl_xor(Ew, off);
-                    Ew = e2 < Ew ? e2 : Ew;
-                }
-                if (lane == 0 && Ew < bound) atomicMin(&I.red->minkey, (unsigned long long)key_of_f64(Ew));
+                // every lane walks (the walk shuffles across the wave); a lane past the end
+                // of a small problem repeats lane 0's work and its result is dropped
+                const uint64_t lam0 = (uint64_t)tile * 64u;
+                const uint64_t lam = lam0 + (uint64_t)lane;
+                const bool active = H >= 6 || lam < (1ull << H);
+                const LaneOutF64 o = walk_lane_f64<M, OMAX>((const double*)I.W, (const double*)I.d, n, L, I.B,
+                                                            active ? lam : lam0, dls, dld, I.margin, wb);
+                if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
@@ -462,7 +476,8 @@ int pick_variant(int kind, int want_M, int n) {
 // into about 2^10 .. 2^22 tiles -- L = n - 16, capped at 18 (n = 28: 12, 32: 16, 36 and
 // 40: 18); inside a batch many problems share the GPU, so fewer, larger tiles per problem
 // keep the per-lane set-up small.
-int default_walk_bits(int n, int M, int OMAX, bool single) {
+int default_walk_bits(int kind, int n, int M, int OMAX, bool single) {
+    (void)kind;   // both walks measured best with the same tiling (float n = 34: L = 18)
     int L = single ? n - 16 : n - 10;
     const int cap = single ? 18 : 16;
     if (L > cap) L = cap;
@@ -511,7 +526,7 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         if (p.variant < 0) return set_err(QBX_EINVAL, "problem %d: no kernel for inner_bits=%d", i, wantM);
         p.M = kVariants[p.variant].M;
         const int OMAX = kVariants[p.variant].OMAX;
-        int L = (o.walk_bits < 0) ? default_walk_bits(n, p.M, OMAX, single) : std::min(o.walk_bits, n);
+        int L = (o.walk_bits < 0) ? default_walk_bits(kind, n, p.M, OMAX, single) : std::min(o.walk_bits, n);
         if (L < p.M) L = p.M;
         if (L > p.M + OMAX || L > n)
             return set_err(QBX_EINVAL, "problem %d: walk_bits=%d outside [%d, %d]", i, L, p.M, std::min(n, p.M + OMAX));

@@ -68,7 +68,8 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
     uint64_t bR = ~0ull;
     // lanes see the best energy found so far as their bound, as the device's waves do
     for (uint64_t lam = 0; lam < lanes; ++lam) {
-        LaneOutF64 o = walk_lane_f64<M, OMAX>(W.data(), d.data(), n, L, B, lam, dls.data(), dld.data(), margin, bE);
+        FixedBound fb{bE};
+        LaneOutF64 o = walk_lane_f64<M, OMAX>(W.data(), d.data(), n, L, B, lam, dls.data(), dld.data(), margin, fb);
         if (o.E < bE || (o.E == bE && o.rank < bR)) { bE = o.E; bR = o.rank; }
     }
     *E_out = bE;

@@ -37,8 +37,10 @@
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
 #define QBX_HD __host__ __device__ __forceinline__
+#define QBX_HD_MEMBER __host__ __device__ __forceinline__
 #else
 #define QBX_HD static inline
+#define QBX_HD_MEMBER inline
 #endif
 
 namespace qbx {
@@ -660,12 +662,21 @@ QBX_HD void group_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const f
 }
 
 // dls: FP32 inner deltas (registers); dld: FP64 inner deltas (memory, read on re-walks).
-// `bound`: an exact energy already reached by some state of the problem (+inf if none):
-// blocks whose FP32 minimum exceeds min(lane best, bound) + margin cannot hold the answer.
-template <int M, int OMAX>
+// Source of an upper bound on the answer: refresh() returns an exact energy already reached
+// by some state of the problem (+inf if none); publish(e) offers the lane's exact best.
+// Blocks whose FP32 minimum exceeds min(lane best, bound) + margin cannot hold the answer.
+// The host build uses FixedBound; the device shares a running bound across waves.
+struct FixedBound {
+    double b;
+    QBX_HD_MEMBER double refresh() const { return b; }
+    QBX_HD_MEMBER void publish(double) {}
+};
+
+template <int M, int OMAX, class Bound>
 QBX_HD LaneOutF64 walk_lane_f64(const double* __restrict__ W, const double* __restrict__ d,
                                 int n, int L, int B, uint64_t lam, const float* dls,
-                                const double* __restrict__ dld, double margin, double bound) {
+                                const double* __restrict__ dld, double margin, Bound& src) {
+    double bound = src.refresh();
     static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
     const int O = L - M;
     const uint64_t p = lane_prefix(lam, L, B);
@@ -700,6 +711,12 @@ QBX_HD LaneOutF64 walk_lane_f64(const double* __restrict__ W, const double* __re
                                 parp ^ parity32(gc), bestE, bestR);
             thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
         }
+        // share progress with the rest of the problem: offer the lane's best, take the
+        // problem's running bound (both are exact energies of real states)
+        src.publish(bestE);
+        const double b2 = src.refresh();
+        bound = b2 < bound ? b2 : bound;
+        thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
         c0 += 1u << GROUP_BITS;
         if (c0 >= nblk) break;
         const int i = ctz32(c0);
