@@ -41,7 +41,8 @@ def _run(cmd):
 def build_lib(out: Path = LIB_OUT) -> Path:
     out.parent.mkdir(parents=True, exist_ok=True)
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "qbx.hip")])
+    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "qbx.hip"),
+          str(CSRC / "qbx_server.hip"), "-lpthread"])
     os.replace(tmp, out)
     return out
 

@@ -1,0 +1,138 @@
+// qbx_host.h -- host-side helpers shared by qbx.hip (single / batch) and qbx_server.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include <string>
+
+#include "qbx_device.h"
+
+namespace qbx {
+
+// last error message of the calling thread (defined in qbx.hip)
+int set_err(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::qbx::set_err(QBX_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+typedef void (*walk_fn_t)(const InstDev*, const TileRange*, int, unsigned long long,
+                          unsigned long long*, int);
+
+struct Variant {
+    int kind, M, OMAX;
+    const void* fn;
+};
+
+// available (kind, M) instantiations, largest M first within each kind (each translation
+// unit launches its own copies of the static kernels)
+static const Variant kVariants[] = {
+    {KIND_I32, 7, 13, (const void*)qbx_walk_i32<7, 13>},
+    {KIND_I32, 6, 14, (const void*)qbx_walk_i32<6, 14>},
+    {KIND_I32, 4, 16, (const void*)qbx_walk_i32<4, 16>},
+    {KIND_I32, 1, 19, (const void*)qbx_walk_i32<1, 19>},
+    {KIND_F64, 6, 14, (const void*)qbx_walk_f64<6, 14>},
+    {KIND_F64, 4, 16, (const void*)qbx_walk_f64<4, 16>},
+    {KIND_F64, 1, 19, (const void*)qbx_walk_f64<1, 19>},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+constexpr int MAX_N = 56;
+constexpr int MAX_LANE_BITS = 37;   // tiles = 2^(H-6) < 2^31
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// largest available M <= min(want_M, n) for this kind, or -1
+static inline int pick_variant(int kind, int want_M, int n) {
+    int best = -1;
+    for (int v = 0; v < kNumVariants; ++v) {
+        if (kVariants[v].kind != kind) continue;
+        if (kVariants[v].M <= want_M && kVariants[v].M <= n) {
+            if (best < 0 || kVariants[v].M > kVariants[best].M) best = v;
+        }
+    }
+    return best;
+}
+
+enum PlanMode { PLAN_SINGLE = 0, PLAN_BATCH = 1 };
+
+// Default walk bits per lane (measured on MI355X): a lone problem is cut into many tiles --
+// L = n - 16, capped at 18 (n = 28: 12, 32: 16, 36 and 40: 18; the float walk measured best
+// with the same rule); inside a batch or a serving step many problems share the GPU, so
+// fewer, larger tiles per problem keep the per-lane set-up small.
+static inline int default_walk_bits(int n, int M, int OMAX, PlanMode mode) {
+    const bool single = mode == PLAN_SINGLE;
+    int L = single ? n - 16 : n - 10;
+    const int cap = single ? 18 : 16;
+    if (L > cap) L = cap;
+    if (L < M) L = M;
+    if (L < n - MAX_LANE_BITS) L = n - MAX_LANE_BITS;
+    if (L > M + OMAX) L = M + OMAX;
+    if (L > n) L = n;
+    return L;
+}
+
+struct Plan {
+    int n, B, L, M, kind, variant, H;
+    uint32_t tiles, tpc, chunks;
+    size_t off_Q, off_W, off_d, off_delta, off_cand, off_slots;
+};
+
+// Tiling of one problem; returns QBX_OK or an error (message set).
+static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mode, int index, Plan& p) {
+    if (n < 1 || n > MAX_N) return set_err(QBX_EINVAL, "problem %d: n=%d outside [1, %d]", index, n, MAX_N);
+    p.n = n;
+    p.kind = kind;
+    p.B = (o.tie_B < 0) ? (n < 20 ? n : 20) : o.tie_B;
+    if (p.B > n) return set_err(QBX_EINVAL, "problem %d: tie_B=%d exceeds n=%d", index, p.B, n);
+    const int wantM = (o.inner_bits < 0) ? (kind == KIND_I32 ? 7 : 6) : o.inner_bits;
+    p.variant = pick_variant(kind, wantM, n);
+    if (p.variant < 0) return set_err(QBX_EINVAL, "problem %d: no kernel for inner_bits=%d", index, wantM);
+    p.M = kVariants[p.variant].M;
+    const int OMAX = kVariants[p.variant].OMAX;
+    int L = (o.walk_bits < 0) ? default_walk_bits(n, p.M, OMAX, mode) : (o.walk_bits < n ? o.walk_bits : n);
+    if (L < p.M) L = p.M;
+    if (L > p.M + OMAX || L > n)
+        return set_err(QBX_EINVAL, "problem %d: walk_bits=%d outside [%d, %d]", index, L, p.M,
+                       (n < p.M + OMAX ? n : p.M + OMAX));
+    if (n - L > MAX_LANE_BITS)
+        return set_err(QBX_EINVAL, "problem %d: walk_bits=%d leaves %d lane bits (max %d)", index, L, n - L,
+                       MAX_LANE_BITS);
+    p.L = L;
+    p.H = n - L;
+    p.tiles = p.H > 6 ? (1u << (p.H - 6)) : 1u;
+    // tiles per chunk: amortise the per-ticket work (atomics, table loads, the final re-walk)
+    // over about 2^22 states, but keep enough chunks to spread one problem over the GPU
+    // (single solves) or over a few waves (batches and serving steps)
+    uint32_t tpc = (22 - 6 - L) > 0 ? (1u << (22 - 6 - L)) : 1u;
+    const uint32_t min_chunks = mode == PLAN_SINGLE ? 8192u : 16u;
+    const uint32_t cap = p.tiles / min_chunks;
+    if (tpc > cap) tpc = cap;
+    if (tpc < 1u) tpc = 1u;
+    p.tpc = tpc;
+    p.chunks = (p.tiles + tpc - 1) / tpc;
+    return QBX_OK;
+}
+
+// log2 of the states one chunk of this plan walks (LPT ordering key)
+static inline int chunk_work_log2(const Plan& p) {
+    return p.L + 6 + (31 - __builtin_clz(p.tpc));
+}
+
+}  // namespace qbx
+
+struct qbx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    char* dbuf = nullptr;
+    size_t dcap = 0;
+    char* hbuf = nullptr;   // pinned staging
+    size_t hcap = 0;
+    int blocks_per_cu[qbx::kNumVariants];
+};
+

@@ -13,8 +13,8 @@ from pathlib import Path
 import numpy as np
 
 __all__ = [
-    "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo",
-    "load_library", "get_context",
+    "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
+    "ServerStats", "load_library", "get_context",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libqbx.so"
@@ -77,6 +77,46 @@ class DevInfo(ctypes.Structure):
     ]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("max_n", ctypes.c_int32),
+        ("max_inflight", ctypes.c_int32),
+        ("work_budget", ctypes.c_double),
+        ("step_states", ctypes.c_double),
+        ("walk_bits", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("E_int", ctypes.c_int64),
+        ("E", ctypes.c_double),
+        ("state", ctypes.c_uint64),
+        ("rank", ctypes.c_uint64),
+        ("t_submit_ms", ctypes.c_double),
+        ("t_admit_ms", ctypes.c_double),
+        ("t_done_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("steps", ctypes.c_uint64),
+        ("chunks", ctypes.c_uint64),
+        ("admissions", ctypes.c_uint64),
+        ("busy_ms", ctypes.c_double),
+    ]
+
+
 # every symbol declared in include/qbx.h, with its ctypes signature
 _P = ctypes.POINTER
 _SIGNATURES = {
@@ -90,6 +130,15 @@ _SIGNATURES = {
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
     "qbx_batch_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
+    "qbx_server_start": (ctypes.c_int, [ctypes.c_void_p, _P(ServerOpts), _P(ctypes.c_void_p)]),
+    "qbx_server_submit_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, ctypes.c_int,
+                                             _P(ctypes.c_uint64)]),
+    "qbx_server_submit_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                             _P(ctypes.c_uint64)]),
+    "qbx_server_poll": (ctypes.c_int, [ctypes.c_void_p, _P(RequestResult), ctypes.c_int, ctypes.c_int,
+                                       _P(ctypes.c_int)]),
+    "qbx_server_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ServerStats)]),
+    "qbx_server_stop": (ctypes.c_int, [ctypes.c_void_p]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 

@@ -1,0 +1,94 @@
+"""Continuous-batching server: solve requests as they arrive, sharing the GPU.
+
+    with Server() as srv:
+        rid = srv.submit(problem_or_Q)
+        ...
+        for res in srv.poll(wait_ms=10): ...
+
+The native worker (libqbx.so, qbx_server_*) admits requests into device slots while the
+in-flight work (sum of 2^n) fits a budget and runs scheduling steps -- finite persistent
+launches over chunks of all admitted requests -- so requests arriving while others run
+are solved alongside them.  Every answer equals the one `QBF().solve` / `engine.solve_matrix`
+gives for the same matrix (same kernels, same tie rule).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native, engine
+
+_P = ctypes.POINTER
+
+
+class Server:
+    def __init__(self, device: int = 0, max_n: int = 40, max_inflight: int = 256,
+                 work_budget: float = 2.0 ** 40, step_states: float = 2.0 ** 34, walk_bits=None):
+        self.ctx = _native.get_context(device)
+        self.lib = self.ctx.lib
+        o = _native.ServerOpts(int(max_n), int(max_inflight), float(work_budget), float(step_states),
+                               -1 if walk_bits is None else int(walk_bits), 0)
+        self._h = ctypes.c_void_p()
+        _native._check(self.lib, self.lib.qbx_server_start(self.ctx._h, ctypes.byref(o), ctypes.byref(self._h)))
+        self.max_n = int(max_n)
+        self._kinds = {}
+        self._buf = (_native.RequestResult * 1024)()
+
+    def submit(self, problem, tie_B=None) -> int:
+        """Queue one problem (a ProblemABC or a square matrix); returns its request id."""
+        Q = problem if isinstance(problem, (np.ndarray, engine.Prepared)) else problem.build_Q_matrix(use_gpu=False)
+        p = Q if isinstance(Q, engine.Prepared) else engine.prepare(Q)
+        if p.n > self.max_n:
+            raise ValueError(f"N={p.n} exceeds the server's max_n={self.max_n}")
+        rid = ctypes.c_uint64()
+        tb = -1 if tie_B is None else int(tie_B)
+        if p.kind == "i32":
+            m = np.ascontiguousarray(p.matrix, dtype=np.int32)
+            rc = self.lib.qbx_server_submit_i32(self._h, m.ctypes.data_as(_P(ctypes.c_int32)), p.n, tb, ctypes.byref(rid))
+        else:
+            m = np.ascontiguousarray(p.matrix, dtype=np.float64)
+            rc = self.lib.qbx_server_submit_f64(self._h, m.ctypes.data_as(_P(ctypes.c_double)), p.n, tb, ctypes.byref(rid))
+        _native._check(self.lib, rc)
+        self._kinds[rid.value] = p
+        return rid.value
+
+    def poll(self, wait_ms: int = 0, max_results: int = 1024) -> list:
+        """Completed requests as dicts: id, E (int / float as the problem), state, rank,
+        status, n and the server-clock times t_submit_ms / t_admit_ms / t_done_ms."""
+        cnt = ctypes.c_int()
+        _native._check(self.lib, self.lib.qbx_server_poll(self._h, self._buf, min(max_results, 1024),
+                                                          int(wait_ms), ctypes.byref(cnt)))
+        out = []
+        for i in range(cnt.value):
+            r = self._buf[i].as_dict()
+            p = self._kinds.pop(r["id"], None)
+            if p is not None:
+                if p.integer:
+                    r["E"] = int(r["E_int"]) if p.kind == "i32" else int(round(r["E"]))
+                elif p.kind == "i32":
+                    r["E"] = float(r["E_int"])
+            out.append(r)
+        return out
+
+    def stats(self) -> dict:
+        s = _native.ServerStats()
+        _native._check(self.lib, self.lib.qbx_server_stats(self._h, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in _native.ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, ctypes.c_void_p()
+            _native._check(self.lib, self.lib.qbx_server_stop(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

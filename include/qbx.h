@@ -100,6 +100,53 @@ int qbx_batch_i32(qbx_ctx_t *ctx, int count, const int32_t *Qpacked, const int64
 int qbx_batch_f64(qbx_ctx_t *ctx, int count, const double *Qpacked, const int64_t *offsets,
                   const int32_t *ns, const qbx_opts_t *opts, qbx_result_t *out);
 
+/* ------------------------------------------------------------------ serving
+ * A continuous-batching server: a native worker thread owns a stream pair and a pool of
+ * device slots.  Requests submitted at any time are admitted when a slot is free and the
+ * in-flight work (sum of 2^n) stays within the budget; every scheduling step is one finite
+ * persistent launch over chunks of all admitted requests (oldest first), so requests that
+ * arrive while others run share the GPU with them.  Results are identical to qbx_solve_*.
+ * No counterpart in the reference, which solves one problem per call (qbf.py:48-70) and
+ * spreads several over processes only through Parallel (parallel.py:124-158). */
+typedef struct qbx_server qbx_server_t;
+
+typedef struct {
+    int32_t max_n;            /* largest n admitted (<= 56); sizes each slot; default 40 */
+    int32_t max_inflight;     /* device slots (admitted requests); default 256 */
+    double work_budget;       /* max sum of 2^n over admitted requests; default 2^40 */
+    double step_states;       /* states scheduled per step launch; default 2^34 */
+    int32_t walk_bits;        /* -1: per-n default */
+    int32_t flags;            /* reserved, 0 */
+} qbx_server_opts_t;
+
+typedef struct {
+    uint64_t id;              /* from qbx_server_submit_* */
+    int32_t status;           /* QBX_OK or a QBX_E* code */
+    int32_t n;
+    int64_t E_int;
+    double E;
+    uint64_t state;
+    uint64_t rank;
+    double t_submit_ms;       /* server clock (ms since qbx_server_start) */
+    double t_admit_ms;
+    double t_done_ms;         /* when the worker saw the result on the host */
+} qbx_request_result_t;
+
+typedef struct {
+    uint64_t submitted, completed, steps, chunks, admissions;
+    double busy_ms;           /* sum of step device times (HIP events) */
+} qbx_server_stats_t;
+
+int qbx_server_start(qbx_ctx_t *ctx, const qbx_server_opts_t *opts, qbx_server_t **out);
+/* Copies Q; tie_B = -1 selects min(n, 20).  Never blocks on the GPU. */
+int qbx_server_submit_i32(qbx_server_t *s, const int32_t *Q, int n, int tie_B, uint64_t *id);
+int qbx_server_submit_f64(qbx_server_t *s, const double *Q, int n, int tie_B, uint64_t *id);
+/* Moves up to `max` completed results into out[]; waits up to wait_ms for the first one. */
+int qbx_server_poll(qbx_server_t *s, qbx_request_result_t *out, int max, int wait_ms, int *count);
+int qbx_server_stats(qbx_server_t *s, qbx_server_stats_t *out);
+/* Finishes every submitted request, joins the worker, frees the server. */
+int qbx_server_stop(qbx_server_t *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,79 @@
+"""Continuous-batching server: every answer equals the answer of the same problem solved
+alone, whether requests arrive together, staggered, or while large ones are running."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import KINDS, random_instance
+from qubo_lib_amd import engine
+from qubo_lib_amd.problems import generators as gen
+from qubo_lib_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def drain(srv, count, timeout_s=60.0):
+    got = {}
+    t0 = time.time()
+    while len(got) < count:
+        for r in srv.poll(wait_ms=20):
+            got[r["id"]] = r
+        assert time.time() - t0 < timeout_s, f"only {len(got)} of {count} results"
+    return got
+
+
+def test_server_matches_individual_solves(gpu_ctx):
+    rng = np.random.default_rng(3)
+    Qs = [random_instance(rng, int(rng.integers(1, 25)), KINDS[i % len(KINDS)]) for i in range(150)]
+    Qs += [rng.standard_normal((n, n)).astype(np.float32) for n in (3, 9, 16, 21)]
+    with Server(max_n=40, max_inflight=64) as srv:
+        ids = [srv.submit(q) for q in Qs]
+        got = drain(srv, len(ids))
+    for q, i in zip(Qs, ids):
+        alone = engine.solve_matrix(q)
+        r = got[i]
+        assert r["status"] == 0 and r["state"] == alone.state and r["E"] == alone.E, (q.shape, r)
+        if q.shape[0] <= 16 and np.issubdtype(q.dtype, np.integer):
+            assert (r["E"], r["state"]) == oracle.brute_i64(q)
+        assert r["t_submit_ms"] <= r["t_admit_ms"] <= r["t_done_ms"]
+
+
+def test_server_requests_arriving_while_large_ones_run(gpu_ctx):
+    big = [gen.symmetric_int_qubo(36, s) for s in range(3)]     # ~10 ms of work together
+    small = [gen.maxcut_gnp(int(n), 0.5, seed=int(n)) for n in range(16, 26)]
+    with Server(max_n=40, max_inflight=32, step_states=2.0 ** 31) as srv:
+        ids_big = [srv.submit(q) for q in big]
+        time.sleep(0.002)
+        ids_small = []
+        for q in small:
+            ids_small.append(srv.submit(q))
+            time.sleep(0.0005)
+        got = drain(srv, len(big) + len(small))
+        st = srv.stats()
+    assert st["steps"] > 3 and st["completed"] == len(big) + len(small)
+    for q, i in list(zip(big, ids_big)) + list(zip(small, ids_small)):
+        alone = engine.solve_matrix(q)
+        assert (got[i]["E"], got[i]["state"]) == (alone.E, alone.state)
+    # short requests that arrived while the long ones ran were served alongside them,
+    # not held behind them
+    done_big = max(got[i]["t_done_ms"] for i in ids_big)
+    early = [i for i in ids_small if got[i]["t_submit_ms"] < done_big - 2.0]
+    assert early and all(got[i]["t_done_ms"] < done_big for i in early)
+
+
+def test_server_budget_and_slots_limit_admission(gpu_ctx):
+    rng = np.random.default_rng(4)
+    Qs = [rng.integers(-50, 50, (22, 22)) for _ in range(40)]
+    with Server(max_n=24, max_inflight=4, work_budget=2.0 ** 23) as srv:
+        ids = [srv.submit(q) for q in Qs]
+        got = drain(srv, len(ids))
+    for q, i in zip(Qs, ids):
+        assert (got[i]["E"], got[i]["state"]) == oracle.brute_i64(q)
+
+
+def test_server_rejects_oversized(gpu_ctx):
+    with Server(max_n=20) as srv:
+        with pytest.raises(ValueError):
+            srv.submit(np.zeros((21, 21), dtype=np.int32))
