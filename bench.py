@@ -294,7 +294,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the prefix space would shard across GPUs with one 8-byte min-reduction; see DESIGN.md)",
+              "(the prefix space would shard across GPUs with one 8-byte min-reduction per problem)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "copy":

@@ -29,7 +29,8 @@
 //     bits and "upper fields" u_q of the outer bits below J move by +-W[.][J].
 //   * a block is re-walked (to find its lowest-rank minimiser) only when its minimum
 //     beats the lane's best in (E, rank) order -- rare after the first few blocks.
-// See DESIGN.md for the derivation of the block key K and of rank_lo().
+//   * ordering: block_key() gives rank bits [M, L) of a block's states and rank_lo() the
+//     inner bits [0, M), so (E, lane, key, rank_lo) orders states as (E, rank) does.
 #pragma once
 #include <stdint.h>
 #include <utility>
