@@ -62,6 +62,57 @@ def cpu_baseline(Q, seconds_target: float = 12.0) -> dict:
     }
 
 
+def _parallel_worker(job):
+    """One Parallel-style partition: the top m bits fixed to `prefix`, the reduced problem
+    solved by the numpy QBF restatement for a bounded number of prefix steps."""
+    Q, m, prefix, seconds_target = job
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle.qbf_numpy import QBFNumpy
+    n = Q.shape[0]
+    x = np.array([(prefix >> (m - 1 - i)) & 1 for i in range(m)], dtype=np.int64)
+    Q = Q.astype(np.int64)
+    R = Q[m:, m:].copy()
+    R[np.diag_indices(n - m)] += x @ Q[:m, m:] + Q[m:, :m] @ x      # fixed-bit fields
+    q = QBFNumpy(R)
+    q.build_cache()
+    _, _, done, sec = q.run(steps=16)
+    steps = int(max(16, min(2 ** q.A, seconds_target / max(sec / max(done, 1), 1e-6))))
+    _, _, done, sec = q.run(steps=steps)
+    return done * 2 ** q.B, sec
+
+
+def cpu_baseline_parallel(Q, seconds_target: float = 6.0) -> dict:
+    """The reference's Parallel(devices=['cpu'] * k) split of the same instance: k processes,
+    each on a reduced problem with the top log2(k) bits fixed (parallel.py:145-157), all
+    running the numpy restatement for a bounded sample; value = total states / slowest."""
+    import multiprocessing as mp
+    k = 1
+    while k * 2 <= min(os.cpu_count() or 1, 16):
+        k *= 2
+    m = k.bit_length() - 1
+    jobs = [(Q, m, p, seconds_target) for p in range(k)]
+    with mp.get_context("fork").Pool(k) as pool:
+        out = pool.map(_parallel_worker, jobs)
+    states = sum(o[0] for o in out)
+    sec = max(o[1] for o in out)
+    return {"value": states / sec, "unit": "states/s", "cores": k,
+            "sample": (f"{k} processes, each the numpy QBF restatement on a reduced n={Q.shape[0] - m} problem "
+                       f"(top {m} bits fixed), {states:.3e} states in {sec:.2f} s (slowest process); "
+                       f"cache builds excluded")}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per walk dispatch from the committed PMC summary (profiles/traffic.json):
+    (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) KiB, or None."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    t = json.loads(path.read_text()).get(workload)
+    if not t:
+        return None
+    return (t["fetch_kib"] * t["factor_fetch"] + t["write_kib"] * t["factor_write"]) * 1024.0
+
+
 def exhaustive(args, ctx, devinfo):
     from qubo_lib_amd import engine
     from qubo_lib_amd.problems.generators import symmetric_int_qubo, symmetric_normal_qubo
@@ -103,6 +154,8 @@ def exhaustive(args, ctx, devinfo):
         g = json.loads(gold.read_text()).get(key)
         if g is not None:
             parity["oracle_state"] = g["state"]
+            parity["oracle_E"] = g["E"]
+            parity["abs_err_vs_oracle"] = abs(float(g["E"]) - float(E))
             parity["matches_oracle"] = bool(g["state"] == state and
                                             (g["E"] == E if isinstance(E, int) else abs(g["E"] - E) < 1e-9 * max(1, abs(E))))
     line = {
@@ -121,10 +174,11 @@ def exhaustive(args, ctx, devinfo):
         "config": {"workload": workload, "n": n, "seed": args.seed, "states_per_step": int(states),
                    "walk_bits": r.info["walk_bits"], "inner_bits": r.info["inner_bits"],
                    "lane_bits": r.info["lane_bits"], "tiles": r.info["tiles"], "tie_B": r.tie_B,
-                   "kernel_ms_mean": kernel_ms, "device": devinfo["name"], "compute_units": devinfo["compute_units"],
+                   "kernel_ms_mean": kernel_ms, "tuning": str(ctx.tuning_file.name) if ctx.tuning_file else "built-in rule",
+                   "device": devinfo["name"], "compute_units": devinfo["compute_units"],
                    "clock_mhz": devinfo["clock_khz"] / 1000.0},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": measured_traffic("exhaustive" if args.workload != "float" else "float"),
                      "ops_per_state": OPS_PER_STATE,
                      "note": "kernel time = HIP events around prep+walk on the library stream"},
         "parity": parity,
@@ -132,6 +186,9 @@ def exhaustive(args, ctx, devinfo):
     if args.cpu_baseline:
         log("timing the CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline(Q)
+        if args.workload != "float":
+            log("timing the process-parallel CPU baseline ...")
+            line["cpu_baseline"]["parallel"] = cpu_baseline_parallel(Q)
     else:
         line["cpu_baseline"] = None
     return line
@@ -146,22 +203,37 @@ def batch(args, ctx, devinfo):
     preps = [engine.prepare(q) for q in Qs]
     mats = [p.matrix for p in preps]
     packed = ctx.pack(mats, np.int32)
+    # batched form: the batch shape captured once in a HIP graph, replayed per step
+    plan = ctx.batch_plan([m.shape[0] for m in mats], kind="i32")
+    for _ in range(args.warmup):
+        plan.run(packed=packed)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = plan.run(packed=packed)
+    wall_b = (time.perf_counter() - t0) / args.steps
+    graph_ms = float(np.mean([res[0].kernel_ms]))
+    plan.close()
+    # the same launches issued eagerly (no graph)
     for _ in range(args.warmup):
         ctx.batch(mats, kind="i32", packed=packed)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = ctx.batch(mats, kind="i32", packed=packed)
-    wall_b = (time.perf_counter() - t0) / args.steps
+        res_e = ctx.batch(mats, kind="i32", packed=packed)
+    wall_e = (time.perf_counter() - t0) / args.steps
     # plain form: one C-ABI solve (launch + sync) per problem, same kernels and GPU
     lat = []
-    t0 = time.perf_counter()
     plain = []
-    for m in mats:
+    mats_plain = mats if args.batch_plain else []
+    t0 = time.perf_counter()
+    for m in mats_plain:
         t1 = time.perf_counter()
         plain.append(ctx.solve_i32(m, walk_bits=None))
         lat.append(time.perf_counter() - t1)
     wall_p = time.perf_counter() - t0
-    same = all((a.E_int, a.state) == (b.E_int, b.state) for a, b in zip(res, plain))
+    if not args.batch_plain:          # profiling runs: batched launches only
+        plain, lat, wall_p = res, [float("nan")], float("nan")
+    same = all((a.E_int, a.state) == (b.E_int, b.state) == (e.E_int, e.state)
+               for a, b, e in zip(res, plain, res_e))
     states = float(sum(2.0 ** q.shape[0] for q in Qs))
     return {
         "metric": "batched solves/s (4096-instance uneven mix)",
@@ -172,7 +244,10 @@ def batch(args, ctx, devinfo):
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"batch_mix({len(Qs)}, seed={args.seed}): MaxCut G(n,0.5) n~U{{16..32}} / int TSP 4-5 cities",
-                   "states": states},
+                   "states": states, "form": "one HIP graph replay per step (copy in, prep, persistent walks, copy out)",
+                   "tuning": str(ctx.tuning_file.name) if ctx.tuning_file else "built-in rule"},
+        "graph_ms": graph_ms,
+        "eager": {"solves_per_s": len(Qs) / wall_e, "ms_per_step": wall_e * 1000.0},
         "plain": {"solves_per_s": len(Qs) / wall_p, "wall_s": wall_p, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                   "p99_ms": float(np.percentile(lat, 99) * 1e3)},
         "speedup_vs_plain": wall_p / wall_b,
@@ -290,6 +365,8 @@ def main():
     ap.add_argument("--walk-bits", type=int, default=None)
     ap.add_argument("--inner-bits", type=int, default=None)
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-batch-plain", dest="batch_plain", action="store_false",
+                    help="batch workload: skip the one-launch-per-problem comparison (for profiling)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -43,7 +43,32 @@ int set_err(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
+// Device arena of one batch: [instances | ranges | Q data] (staged from the host in one
+// copy), then the tables qbx_prep writes, the reduction state, the results and the tickets.
+struct Group { int variant; int first; int cnt; unsigned long long total; };
+struct Layout {
+    int kind = 0, count = 0;
+    size_t esize = 0;
+    std::vector<Plan> plans;
+    std::vector<int> order;      // launch order: variant, then longest chunks first
+    std::vector<Group> groups;   // one persistent launch per variant
+    size_t off_inst = 0, off_ranges = 0, stage_bytes = 0, off_red = 0, off_res = 0, off_ticket = 0;
+    size_t total_bytes = 0;
+};
 }  // namespace qbx
+
+// A batch shape (kinds, sizes, tiling) with its own device arena, pinned buffers and the
+// captured graph of its launches; qbx_batch_plan_run_* refills Q and replays the graph.
+struct qbx_batch_plan {
+    qbx_ctx* ctx = nullptr;
+    qbx::Layout lay;
+    char* dv = nullptr;
+    char* h_stage = nullptr;
+    char* h_res = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
 
 namespace {
 
@@ -69,44 +94,42 @@ int ensure_capacity(qbx_ctx* c, size_t dbytes, size_t hbytes) {
     return QBX_OK;
 }
 
-template <class T>
-int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* offsets,
-              const int32_t* ns, const qbx_opts_t* opts, qbx_result_t* out, bool single) {
-    if (!c) return set_err(QBX_EINVAL, "null context");
+int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_opts_t* opts, bool single,
+                Layout& lay) {
     if (count <= 0) return set_err(QBX_EINVAL, "count must be positive, got %d", count);
-    if (!Qpacked || !offsets || !ns || !out) return set_err(QBX_EINVAL, "null argument");
+    if (!ns) return set_err(QBX_EINVAL, "null argument");
     qbx_opts_t o = {-1, -1, -1, 0};
     if (opts) o = *opts;
     if (o.flags != 0) return set_err(QBX_EINVAL, "opts.flags must be 0");
-    const size_t esize = sizeof(T);
-
-    std::vector<Plan> plans(count);
+    lay.kind = kind;
+    lay.count = count;
+    lay.esize = kind == KIND_I32 ? sizeof(int32_t) : sizeof(double);
+    lay.plans.assign(count, Plan());
+    std::vector<Plan>& plans = lay.plans;
     size_t off = 0;
-    const size_t off_inst = off;
+    lay.off_inst = off;
     off = align256(off + sizeof(InstDev) * count);
-    const size_t off_ranges = off;
+    lay.off_ranges = off;
     off = align256(off + sizeof(TileRange) * count);
     const PlanMode mode = single ? PLAN_SINGLE : PLAN_BATCH;
     for (int i = 0; i < count; ++i) {
-        const int n = ns[i];
-        if (offsets[i] < 0) return set_err(QBX_EINVAL, "problem %d: negative offset", i);
         Plan& p = plans[i];
-        const int rc = plan_problem(kind, n, o, mode, i, p);
+        const int rc = plan_problem(kind, ns[i], o, mode, i, p, &c->walk);
         if (rc) return rc;
         p.off_Q = off;
-        off = align256(off + esize * (size_t)n * n);
+        off = align256(off + lay.esize * (size_t)p.n * p.n);
     }
-    const size_t stage_bytes = off;   // [instances | ranges | Q data] staged from host
+    lay.stage_bytes = off;
     for (int i = 0; i < count; ++i) {
         Plan& p = plans[i];
         p.off_W = off;
-        off = align256(off + esize * (size_t)p.n * LDW);
+        off = align256(off + lay.esize * (size_t)p.n * LDW);
         p.off_d = off;
-        off = align256(off + esize * 64);
+        off = align256(off + lay.esize * 64);
         p.off_delta = off;
         off = align256(off + (kind == KIND_I32 ? 4 : 12) * (size_t)(1u << p.M));
     }
-    const size_t off_red = off;
+    lay.off_red = off;
     off = align256(off + sizeof(RedDev) * count);
     for (int i = 0; i < count; ++i) {
         plans[i].off_cand = off;
@@ -116,88 +139,108 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         plans[i].off_slots = off;
         off = align256(off + sizeof(SlotDev) * (size_t)plans[i].chunks);
     }
-    const size_t off_res = off;
+    lay.off_res = off;
     off = align256(off + sizeof(ResultDev) * count);
-    const size_t off_ticket = off;
+    lay.off_ticket = off;
     off = align256(off + sizeof(unsigned long long) * MAX_GROUPS);
-    const size_t total_bytes = off;
+    lay.total_bytes = off;
 
-    int rc = ensure_capacity(c, total_bytes, std::max(stage_bytes, sizeof(ResultDev) * (size_t)count));
-    if (rc) return rc;
-    char* h = c->hbuf;
-    char* dv = c->dbuf;
-
-    // order: group by kernel variant, then longest tiles first (LPT), then problem index
-    std::vector<int> order(count);
-    for (int i = 0; i < count; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    lay.order.resize(count);
+    for (int i = 0; i < count; ++i) lay.order[i] = i;
+    std::stable_sort(lay.order.begin(), lay.order.end(), [&](int a, int b) {
         if (plans[a].variant != plans[b].variant) return plans[a].variant < plans[b].variant;
-        return chunk_work_log2(plans[a]) > chunk_work_log2(plans[b]);   // longest chunks first
+        return chunk_work_log2(plans[a]) > chunk_work_log2(plans[b]);
     });
+    lay.groups.clear();
+    for (int r = 0; r < count; ++r) {
+        const int i = lay.order[r];
+        if (lay.groups.empty() || lay.groups.back().variant != plans[i].variant)
+            lay.groups.push_back({plans[i].variant, r, 0, 0});
+        Group& G = lay.groups.back();
+        G.cnt++;
+        G.total += plans[i].chunks;
+    }
+    if ((int)lay.groups.size() > MAX_GROUPS) return set_err(QBX_EINVAL, "too many kernel variants in one batch");
+    return QBX_OK;
+}
 
-    InstDev* hinst = (InstDev*)(h + off_inst);
-    TileRange* hr = (TileRange*)(h + off_ranges);
-    for (int i = 0; i < count; ++i) {
-        const Plan& p = plans[i];
+// Instance descriptors and tile ranges (device pointers into the arena at dv) into the
+// host staging buffer h.
+void stage_descriptors(const Layout& lay, char* h, char* dv) {
+    InstDev* hinst = (InstDev*)(h + lay.off_inst);
+    for (int i = 0; i < lay.count; ++i) {
+        const Plan& p = lay.plans[i];
         InstDev& I = hinst[i];
         memset(&I, 0, sizeof I);
-        I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = kind; I.status = 0; I.tiles = p.tiles;
+        I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = lay.kind; I.status = 0; I.tiles = p.tiles;
         I.tpc = p.tpc; I.chunks = p.chunks;
         I.Q = dv + p.off_Q;
         I.W = dv + p.off_W;
         I.d = dv + p.off_d;
         I.delta = dv + p.off_delta;
-        I.red = (RedDev*)(dv + off_red) + i;
+        I.red = (RedDev*)(dv + lay.off_red) + i;
         I.cand = (uint32_t*)(dv + p.off_cand);
         I.slots = (SlotDev*)(dv + p.off_slots);
-        I.result = (ResultDev*)(dv + off_res) + i;
-        memcpy(h + p.off_Q, Qpacked + offsets[i], esize * (size_t)p.n * p.n);
+        I.result = (ResultDev*)(dv + lay.off_res) + i;
     }
-    // one range per problem; groups of equal variant are contiguous in `order`
-    struct Group { int variant; int first; int cnt; unsigned long long total; };
-    std::vector<Group> groups;
+    TileRange* hr = (TileRange*)(h + lay.off_ranges);
+    // tickets count from 0 within each variant's launch
     unsigned long long prefix = 0;
-    for (int r = 0; r < count; ++r) {
-        const int i = order[r];
-        if (groups.empty() || groups.back().variant != plans[i].variant) {
-            groups.push_back({plans[i].variant, r, 0, 0});
+    size_t next_group = 0;
+    for (int r = 0; r < lay.count; ++r) {
+        const int i = lay.order[r];
+        if (next_group < lay.groups.size() && lay.groups[next_group].first == r) {
             prefix = 0;
+            ++next_group;
         }
         TileRange& R = hr[r];
-        R.inst = (uint32_t)i; R.c_begin = 0; R.c_count = plans[i].chunks; R.pad = 0; R.prefix = prefix;
-        prefix += plans[i].chunks;
-        groups.back().cnt++;
-        groups.back().total = prefix;
+        R.inst = (uint32_t)i; R.c_begin = 0; R.c_count = lay.plans[i].chunks; R.pad = 0; R.prefix = prefix;
+        prefix += lay.plans[i].chunks;
     }
-    if ((int)groups.size() > MAX_GROUPS) return set_err(QBX_EINVAL, "too many kernel variants in one batch");
+}
 
-    HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipMemcpyAsync(dv, h, stage_bytes, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    unsigned long long* dticket = (unsigned long long*)(dv + off_ticket);
-    hipLaunchKernelGGL(qbx_prep, dim3(count), dim3(256), 0, c->stream, (InstDev*)(dv + off_inst),
+template <class T>
+int stage_matrices(const Layout& lay, char* h, const T* Qpacked, const int64_t* offsets) {
+    if (!Qpacked || !offsets) return set_err(QBX_EINVAL, "null argument");
+    for (int i = 0; i < lay.count; ++i) {
+        if (offsets[i] < 0) return set_err(QBX_EINVAL, "problem %d: negative offset", i);
+        const Plan& p = lay.plans[i];
+        memcpy(h + p.off_Q, Qpacked + offsets[i], lay.esize * (size_t)p.n * p.n);
+    }
+    return QBX_OK;
+}
+
+// The work of one batch on `stream`: staged copy in, table build, one persistent launch per
+// variant, results out.  Issues no memset and no allocation, so it can be captured in a
+// graph; ev_begin / ev_end (nullable, not used under capture) bracket the kernels.
+int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const char* h_stage, char* h_res,
+            hipEvent_t ev_begin, hipEvent_t ev_end) {
+    HIP_TRY(hipMemcpyAsync(dv, h_stage, lay.stage_bytes, hipMemcpyHostToDevice, stream));
+    if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, stream));
+    unsigned long long* dticket = (unsigned long long*)(dv + lay.off_ticket);
+    hipLaunchKernelGGL(qbx_prep, dim3(lay.count), dim3(256), 0, stream, (InstDev*)(dv + lay.off_inst),
                        (const uint32_t*)nullptr, dticket, (const InstDev*)nullptr);
     HIP_TRY(hipGetLastError());
-    for (size_t g = 0; g < groups.size(); ++g) {
-        const Group& G = groups[g];
+    for (size_t g = 0; g < lay.groups.size(); ++g) {
+        const Group& G = lay.groups[g];
         const int v = G.variant;
         const unsigned long long grid_full = (unsigned long long)c->blocks_per_cu[v] * c->prop.multiProcessorCount;
         const unsigned long long grid = std::max(1ull, std::min(grid_full, G.total));
         walk_fn_t fn = (walk_fn_t)kVariants[v].fn;
-        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64), 0, c->stream,
-                           (const InstDev*)(dv + off_inst), (const TileRange*)(dv + off_ranges) + G.first,
+        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64), 0, stream,
+                           (const InstDev*)(dv + lay.off_inst), (const TileRange*)(dv + lay.off_ranges) + G.first,
                            G.cnt, G.total, dticket + g, 0);
         HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    HIP_TRY(hipMemcpyAsync(h, dv + off_res, sizeof(ResultDev) * count, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    const ResultDev* hres = (const ResultDev*)h;
+    if (ev_end) HIP_TRY(hipEventRecord(ev_end, stream));
+    HIP_TRY(hipMemcpyAsync(h_res, dv + lay.off_res, sizeof(ResultDev) * lay.count, hipMemcpyDeviceToHost, stream));
+    return QBX_OK;
+}
+
+int collect(const Layout& lay, const ResultDev* hres, float ms, qbx_result_t* out) {
     int first_bad = QBX_OK;
-    for (int i = 0; i < count; ++i) {
-        const Plan& p = plans[i];
+    for (int i = 0; i < lay.count; ++i) {
+        const Plan& p = lay.plans[i];
         qbx_result_t& R = out[i];
         memset(&R, 0, sizeof R);
         R.n = p.n; R.tie_B = p.B; R.walk_bits = p.L; R.inner_bits = p.M; R.lane_bits = p.H;
@@ -207,7 +250,7 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         R.status = hres[i].status;
         R.rank = hres[i].rank;
         R.state = hres[i].state;
-        if (kind == KIND_I32) {
+        if (lay.kind == KIND_I32) {
             R.E_int = (int64_t)hres[i].Ebits;
             R.E = (double)R.E_int;
         } else {
@@ -219,10 +262,88 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         if (R.status != 0 && first_bad == QBX_OK) {
             first_bad = R.status;
             set_err(R.status, "problem %d: energy bound outside the %s path", i,
-                    kind == KIND_I32 ? "int32" : "float");
+                    lay.kind == KIND_I32 ? "int32" : "float");
         }
     }
     return first_bad;
+}
+
+template <class T>
+int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* offsets,
+              const int32_t* ns, const qbx_opts_t* opts, qbx_result_t* out, bool single) {
+    if (!c) return set_err(QBX_EINVAL, "null context");
+    if (!Qpacked || !offsets || !ns || !out) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    int rc = make_layout(c, kind, count, ns, opts, single, lay);
+    if (rc) return rc;
+    rc = ensure_capacity(c, lay.total_bytes, std::max(lay.stage_bytes, sizeof(ResultDev) * (size_t)count));
+    if (rc) return rc;
+    char* h = c->hbuf;
+    char* dv = c->dbuf;
+    stage_descriptors(lay, h, dv);
+    rc = stage_matrices(lay, h, Qpacked, offsets);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    rc = enqueue(c, c->stream, lay, dv, h, h, c->ev0, c->ev1);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return collect(lay, (const ResultDev*)h, ms, out);
+}
+
+void plan_free(qbx_batch_plan* P) {
+    if (!P) return;
+    if (P->ctx) hipSetDevice(P->ctx->device);
+    if (P->exec) hipGraphExecDestroy(P->exec);
+    if (P->graph) hipGraphDestroy(P->graph);
+    if (P->dv) hipFree(P->dv);
+    if (P->h_stage) hipHostFree(P->h_stage);
+    if (P->h_res) hipHostFree(P->h_res);
+    if (P->ev0) hipEventDestroy(P->ev0);
+    if (P->ev1) hipEventDestroy(P->ev1);
+    delete P;
+}
+
+int plan_build(qbx_batch_plan* P) {
+    qbx_ctx* c = P->ctx;
+    const Layout& lay = P->lay;
+    HIP_TRY(hipSetDevice(c->device));
+    if (hipMalloc((void**)&P->dv, lay.total_bytes) != hipSuccess)
+        return set_err(QBX_ENOMEM, "hipMalloc(%zu) failed", lay.total_bytes);
+    if (hipHostMalloc((void**)&P->h_stage, lay.stage_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&P->h_res, sizeof(ResultDev) * lay.count, hipHostMallocDefault) != hipSuccess)
+        return set_err(QBX_ENOMEM, "hipHostMalloc failed");
+    memset(P->h_stage, 0, lay.stage_bytes);
+    stage_descriptors(lay, P->h_stage, P->dv);
+    HIP_TRY(hipEventCreate(&P->ev0));
+    HIP_TRY(hipEventCreate(&P->ev1));
+    // the host thread owns the context stream, so nothing else is queued on it meanwhile
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(c, c->stream, lay, P->dv, P->h_stage, P->h_res, nullptr, nullptr);
+    const hipError_t e = hipStreamEndCapture(c->stream, &P->graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(QBX_EDEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    return QBX_OK;
+}
+
+template <class T>
+int plan_run(qbx_batch_plan* P, int kind, const T* Qpacked, const int64_t* offsets, qbx_result_t* out) {
+    if (!P || !out) return set_err(QBX_EINVAL, "null argument");
+    if (P->lay.kind != kind)
+        return set_err(QBX_EINVAL, "plan was made for %s matrices", P->lay.kind == KIND_I32 ? "int32" : "double");
+    int rc = stage_matrices(P->lay, P->h_stage, Qpacked, offsets);
+    if (rc) return rc;
+    qbx_ctx* c = P->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(P->ev0, c->stream));
+    HIP_TRY(hipGraphLaunch(P->exec, c->stream));
+    HIP_TRY(hipEventRecord(P->ev1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+    return collect(P->lay, (const ResultDev*)P->h_res, ms, out);
 }
 
 }  // namespace
@@ -239,6 +360,7 @@ int qbx_open(int device, qbx_ctx_t** out) {
     if (device < 0 || device >= cnt) return set_err(QBX_EINVAL, "device %d not present (%d devices)", device, cnt);
     qbx_ctx* c = new qbx_ctx();
     c->device = device;
+    c->walk.clear();
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipGetDeviceProperties(&c->prop, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -294,6 +416,69 @@ int qbx_solve_f64(qbx_ctx_t* c, const double* Q, int n, const qbx_opts_t* opts, 
     const int64_t off = 0;
     const int32_t nn = n;
     return run_batch<double>(c, KIND_F64, 1, Q, &off, &nn, opts, out, true);
+}
+
+int qbx_set_walk_table(qbx_ctx_t* c, int kind, int mode, const int32_t* walk_bits, int count) {
+    if (!c) return set_err(QBX_EINVAL, "null context");
+    if (kind != QBX_KIND_I32 && kind != QBX_KIND_F64) return set_err(QBX_EINVAL, "kind must be 0 or 1");
+    if (mode != QBX_MODE_SINGLE && mode != QBX_MODE_BATCH) return set_err(QBX_EINVAL, "mode must be 0 or 1");
+    if (count < 0 || count > MAX_N + 1 || (count > 0 && !walk_bits))
+        return set_err(QBX_EINVAL, "count must be in [0, %d] with a non-null table", MAX_N + 1);
+    for (int n = 0; n < count; ++n)
+        if (walk_bits[n] > MAX_N) return set_err(QBX_EINVAL, "walk_bits[%d]=%d exceeds %d", n, walk_bits[n], MAX_N);
+    for (int n = 0; n <= MAX_N; ++n)
+        c->walk.L[kind][mode][n] = (int8_t)((n < count && walk_bits[n] >= 0) ? walk_bits[n] : -1);
+    return QBX_OK;
+}
+
+int qbx_plan_info(qbx_ctx_t* c, int kind, int mode, int n, const qbx_opts_t* opts, qbx_result_t* out) {
+    if (!c || !out) return set_err(QBX_EINVAL, "null argument");
+    if (kind != QBX_KIND_I32 && kind != QBX_KIND_F64) return set_err(QBX_EINVAL, "kind must be 0 or 1");
+    if (mode != QBX_MODE_SINGLE && mode != QBX_MODE_BATCH) return set_err(QBX_EINVAL, "mode must be 0 or 1");
+    qbx_opts_t o = {-1, -1, -1, 0};
+    if (opts) o = *opts;
+    Plan p;
+    const int rc = plan_problem(kind, n, o, mode == QBX_MODE_SINGLE ? PLAN_SINGLE : PLAN_BATCH, 0, p, &c->walk);
+    if (rc) return rc;
+    memset(out, 0, sizeof *out);
+    out->n = p.n; out->tie_B = p.B; out->walk_bits = p.L; out->inner_bits = p.M; out->lane_bits = p.H;
+    out->states = (p.n >= 64) ? ~0ull : (1ull << p.n);
+    out->tiles = p.tiles;
+    return QBX_OK;
+}
+
+int qbx_batch_plan_create(qbx_ctx_t* c, int kind, int count, const int32_t* ns, const qbx_opts_t* opts,
+                          qbx_batch_plan_t** out) {
+    if (!c || !out) return set_err(QBX_EINVAL, "null argument");
+    *out = nullptr;
+    if (kind != QBX_KIND_I32 && kind != QBX_KIND_F64) return set_err(QBX_EINVAL, "kind must be 0 or 1");
+    qbx_batch_plan* P = new qbx_batch_plan();
+    P->ctx = c;
+    int rc = make_layout(c, kind, count, ns, opts, false, P->lay);
+    if (rc == QBX_OK) rc = plan_build(P);
+    if (rc != QBX_OK) {
+        plan_free(P);
+        return rc;
+    }
+    *out = P;
+    return QBX_OK;
+}
+
+int qbx_batch_plan_run_i32(qbx_batch_plan_t* P, const int32_t* Qpacked, const int64_t* offsets, qbx_result_t* out) {
+    return plan_run<int32_t>(P, KIND_I32, Qpacked, offsets, out);
+}
+
+int qbx_batch_plan_run_f64(qbx_batch_plan_t* P, const double* Qpacked, const int64_t* offsets, qbx_result_t* out) {
+    return plan_run<double>(P, KIND_F64, Qpacked, offsets, out);
+}
+
+int qbx_batch_plan_destroy(qbx_batch_plan_t* P) {
+    if (P && P->ctx) {
+        hipSetDevice(P->ctx->device);
+        hipStreamSynchronize(P->ctx->stream);
+    }
+    plan_free(P);
+    return QBX_OK;
 }
 
 int qbx_batch_i32(qbx_ctx_t* c, int count, const int32_t* Qpacked, const int64_t* offsets,

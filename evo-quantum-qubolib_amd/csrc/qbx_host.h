@@ -75,6 +75,17 @@ static inline int default_walk_bits(int n, int M, int OMAX, PlanMode mode) {
     return L;
 }
 
+// Per-n walk bits chosen by an autotuner (qbx_set_walk_table), indexed
+// [kind][mode][n]; entries < 0 fall back to default_walk_bits().
+struct WalkTable {
+    int8_t L[2][2][MAX_N + 1];
+    void clear() {
+        for (auto& a : L)
+            for (auto& b : a)
+                for (auto& v : b) v = -1;
+    }
+};
+
 struct Plan {
     int n, B, L, M, kind, variant, H;
     uint32_t tiles, tpc, chunks;
@@ -82,7 +93,8 @@ struct Plan {
 };
 
 // Tiling of one problem; returns QBX_OK or an error (message set).
-static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mode, int index, Plan& p) {
+static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mode, int index, Plan& p,
+                               const WalkTable* table = nullptr) {
     if (n < 1 || n > MAX_N) return set_err(QBX_EINVAL, "problem %d: n=%d outside [1, %d]", index, n, MAX_N);
     p.n = n;
     p.kind = kind;
@@ -93,7 +105,15 @@ static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mo
     if (p.variant < 0) return set_err(QBX_EINVAL, "problem %d: no kernel for inner_bits=%d", index, wantM);
     p.M = kVariants[p.variant].M;
     const int OMAX = kVariants[p.variant].OMAX;
-    int L = (o.walk_bits < 0) ? default_walk_bits(n, p.M, OMAX, mode) : (o.walk_bits < n ? o.walk_bits : n);
+    int L;
+    if (o.walk_bits >= 0) {
+        L = o.walk_bits < n ? o.walk_bits : n;
+    } else {
+        L = default_walk_bits(n, p.M, OMAX, mode);
+        const int t = table ? table->L[kind][mode][n] : -1;
+        // a tuned entry is used only where it is valid for the variant picked here
+        if (t >= p.M && t <= p.M + OMAX && t <= n && n - t <= MAX_LANE_BITS) L = t;
+    }
     if (L < p.M) L = p.M;
     if (L > p.M + OMAX || L > n)
         return set_err(QBX_EINVAL, "problem %d: walk_bits=%d outside [%d, %d]", index, L, p.M,
@@ -134,5 +154,6 @@ struct qbx_ctx {
     char* hbuf = nullptr;   // pinned staging
     size_t hcap = 0;
     int blocks_per_cu[qbx::kNumVariants];
+    qbx::WalkTable walk;    // tuned per-n walk bits (all -1 until qbx_set_walk_table)
 };
 

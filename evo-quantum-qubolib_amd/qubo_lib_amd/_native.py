@@ -14,10 +14,11 @@ import numpy as np
 
 __all__ = [
     "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
-    "ServerStats", "load_library", "get_context",
+    "ServerStats", "BatchPlan", "KINDS", "MODES", "load_library", "get_context",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libqbx.so"
+TUNING_DIR = Path(__file__).resolve().parent / "tuning"
 
 QBX_OK = 0
 QBX_EINVAL = -1
@@ -25,6 +26,9 @@ QBX_ERANGE = -2
 QBX_EDEVICE = -3
 QBX_ENOMEM = -4
 QBX_EBUSY = -5
+
+KINDS = {"i32": 0, "f64": 1}
+MODES = {"single": 0, "batch": 1}
 
 
 class NativeError(RuntimeError):
@@ -130,6 +134,17 @@ _SIGNATURES = {
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
     "qbx_batch_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
+    "qbx_set_walk_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int32),
+                                          ctypes.c_int]),
+    "qbx_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(Opts),
+                                     _P(Result)]),
+    "qbx_batch_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int32),
+                                             _P(Opts), _P(ctypes.c_void_p)]),
+    "qbx_batch_plan_run_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int64),
+                                              _P(Result)]),
+    "qbx_batch_plan_run_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), _P(ctypes.c_int64),
+                                              _P(Result)]),
+    "qbx_batch_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "qbx_server_start": (ctypes.c_int, [ctypes.c_void_p, _P(ServerOpts), _P(ctypes.c_void_p)]),
     "qbx_server_submit_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, ctypes.c_int,
                                              _P(ctypes.c_uint64)]),
@@ -185,12 +200,57 @@ def _opts(tie_B=None, walk_bits=None, inner_bits=None) -> Opts:
 class Context:
     """One HIP device and stream.  Not thread-safe: one host thread at a time."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, tuning: str | os.PathLike | None = None):
         self.lib = load_library()
         self._h = ctypes.c_void_p()
         _check(self.lib, self.lib.qbx_open(int(device), ctypes.byref(self._h)))
         self.device = int(device)
         self.lock = threading.Lock()
+        self.tuning_file = None
+        self.load_tuning(tuning)
+
+    # -- tiling -----------------------------------------------------------------------
+    def load_tuning(self, path=None) -> Path | None:
+        """Install the per-n walk-bit tables written by qubo_lib_amd.tune for this device's
+        architecture.  Looked up in order: `path`, $QUBO_LIB_AMD_TUNING ("off" disables),
+        qubo_lib_amd/tuning/<arch>.json.  A missing file keeps the built-in rule."""
+        env = os.environ.get("QUBO_LIB_AMD_TUNING")
+        if path is None and env:
+            if env.lower() == "off":
+                return None
+            path = env
+        if path is None:
+            arch = self.device_info()["name"].split(":")[0]
+            path = TUNING_DIR / f"{arch}.json"
+        path = Path(path)
+        if not path.exists():
+            return None
+        import json
+        data = json.loads(path.read_text())
+        for key, table in data.get("walk_bits", {}).items():
+            kind, mode = key.split("/")
+            self.set_walk_table(kind, mode, [int(v) for v in table])
+        self.tuning_file = path
+        return path
+
+    def set_walk_table(self, kind: str, mode: str, table) -> None:
+        """walk_bits per n (index = n, -1 = built-in rule) for one kind ("i32"/"f64") and
+        mode ("single"/"batch")."""
+        arr = np.ascontiguousarray(np.asarray(table, dtype=np.int32))
+        with self.lock:
+            rc = self.lib.qbx_set_walk_table(self._h, KINDS[kind], MODES[mode],
+                                             arr.ctypes.data_as(_P(ctypes.c_int32)), int(arr.size))
+        _check(self.lib, rc)
+
+    def plan_info(self, kind: str, mode: str, n: int, **kw) -> Result:
+        """Tiling a solve of size n would get (walk_bits, inner_bits, lane_bits, tiles)."""
+        out = Result()
+        o = _opts(**kw)
+        with self.lock:
+            rc = self.lib.qbx_plan_info(self._h, KINDS[kind], MODES[mode], int(n), ctypes.byref(o),
+                                        ctypes.byref(out))
+        _check(self.lib, rc)
+        return out
 
     def close(self):
         if self._h:
@@ -262,6 +322,66 @@ class Context:
         if rc not in (QBX_OK, QBX_ERANGE):
             _check(self.lib, rc)
         return list(out)
+
+    def batch_plan(self, ns, kind: str = "i32", **kw) -> "BatchPlan":
+        """Capture the launches of one batch shape in a HIP graph (see BatchPlan)."""
+        return BatchPlan(self, ns, kind, **kw)
+
+
+class BatchPlan:
+    """A batch shape (problem sizes, kind, tiling) with its own device memory and a
+    captured HIP graph; run() copies new matrices of that shape in and replays the graph.
+    Uses the context's stream: one host thread at a time, like the context."""
+
+    def __init__(self, ctx: Context, ns, kind: str = "i32", **kw):
+        self.ctx = ctx
+        self.kind = kind
+        self.ns = np.ascontiguousarray(np.asarray(ns, dtype=np.int32))
+        self.dtype = np.int32 if kind == "i32" else np.float64
+        self._h = ctypes.c_void_p()
+        o = _opts(**kw)
+        with ctx.lock:
+            rc = ctx.lib.qbx_batch_plan_create(ctx._h, KINDS[kind], int(self.ns.size),
+                                               self.ns.ctypes.data_as(_P(ctypes.c_int32)), ctypes.byref(o),
+                                               ctypes.byref(self._h))
+        _check(ctx.lib, rc)
+
+    def run(self, Qs=None, packed=None):
+        """Solve matrices of the planned sizes (a list, or Context.pack output); returns a
+        list of Result."""
+        if packed is None:
+            packed = Context.pack(Qs, self.dtype)
+        data, offsets, ns = packed
+        if not np.array_equal(np.asarray(ns, dtype=np.int32), self.ns):
+            raise ValueError("matrix sizes differ from the planned batch shape")
+        data = np.ascontiguousarray(data, dtype=self.dtype)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        out = (Result * int(self.ns.size))()
+        fn, ptr = ((self.ctx.lib.qbx_batch_plan_run_i32, ctypes.c_int32) if self.kind == "i32"
+                   else (self.ctx.lib.qbx_batch_plan_run_f64, ctypes.c_double))
+        with self.ctx.lock:
+            rc = fn(self._h, data.ctypes.data_as(_P(ptr)), offsets.ctypes.data_as(_P(ctypes.c_int64)), out)
+        if rc not in (QBX_OK, QBX_ERANGE):
+            _check(self.ctx.lib, rc)
+        return list(out)
+
+    def close(self):
+        if self._h:
+            with self.ctx.lock:
+                self.ctx.lib.qbx_batch_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _contexts: dict[int, Context] = {}

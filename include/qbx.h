@@ -32,6 +32,11 @@ extern "C" {
 #define QBX_ENOMEM (-4)     /* allocation failure */
 #define QBX_EBUSY (-5)      /* server queue full / budget exhausted */
 
+#define QBX_KIND_I32 0      /* int32 matrices, exact int32 energies */
+#define QBX_KIND_F64 1      /* double matrices, FP32 filter + FP64 re-walk */
+#define QBX_MODE_SINGLE 0   /* one problem spread over the whole GPU */
+#define QBX_MODE_BATCH 1    /* many problems sharing the GPU (batches, serving) */
+
 typedef struct qbx_ctx qbx_ctx_t;
 
 typedef struct {
@@ -99,6 +104,31 @@ int qbx_batch_i32(qbx_ctx_t *ctx, int count, const int32_t *Qpacked, const int64
                   const int32_t *ns, const qbx_opts_t *opts, qbx_result_t *out);
 int qbx_batch_f64(qbx_ctx_t *ctx, int count, const double *Qpacked, const int64_t *offsets,
                   const int32_t *ns, const qbx_opts_t *opts, qbx_result_t *out);
+
+/* Batch plans: the launches of one batch shape (count, kind, each n, opts) captured once in
+ * a HIP graph with their own device arena; every run copies new matrices of that shape in
+ * and replays the graph (copy in, table build, persistent walks, results out).  Results
+ * are identical to qbx_batch_*; out[i].kernel_ms is the whole replay, copies included.
+ * Same reference counterpart as qbx_batch_* (parallel.py:145-157, 381-385). */
+typedef struct qbx_batch_plan qbx_batch_plan_t;
+int qbx_batch_plan_create(qbx_ctx_t *ctx, int kind, int count, const int32_t *ns, const qbx_opts_t *opts,
+                          qbx_batch_plan_t **plan);
+int qbx_batch_plan_run_i32(qbx_batch_plan_t *plan, const int32_t *Qpacked, const int64_t *offsets,
+                           qbx_result_t *out);
+int qbx_batch_plan_run_f64(qbx_batch_plan_t *plan, const double *Qpacked, const int64_t *offsets,
+                           qbx_result_t *out);
+int qbx_batch_plan_destroy(qbx_batch_plan_t *plan);
+
+/* ------------------------------------------------------------------- tiling
+ * Walk bits (L: states per lane = 2^L, lanes per problem = 2^(n-L)) used when
+ * opts->walk_bits = -1.  qbx_set_walk_table installs a per-n table measured by the
+ * autotuner (qubo_lib_amd.tune) for one kind and mode: walk_bits[n], n < count <= 57;
+ * entries < 0, or invalid for the kernel variant n gets, keep the built-in rule.
+ * qbx_plan_info reports the tiling a solve of size n would get, without running it.
+ * The reference sizes its tiling by memory instead (QBF.define_chunk_size and
+ * adjust_AB_to_fit_memory, qbf.py:430-466; QB_F1's fixed B = 18, qbf1.py:94-95). */
+int qbx_set_walk_table(qbx_ctx_t *ctx, int kind, int mode, const int32_t *walk_bits, int count);
+int qbx_plan_info(qbx_ctx_t *ctx, int kind, int mode, int n, const qbx_opts_t *opts, qbx_result_t *out);
 
 /* ------------------------------------------------------------------ serving
  * A continuous-batching server: a native worker thread owns a stream pair and a pool of
