@@ -119,7 +119,7 @@ def exhaustive(args, ctx, devinfo):
     if args.workload == "float":
         n = 34
         Q = symmetric_normal_qubo(n, args.seed)
-        workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: FP32 filter walk, FP64 exact re-walk"
+        workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: quantized int32 filter walk, FP64 exact re-walk"
         dtype = "f32"
     else:
         n = 40

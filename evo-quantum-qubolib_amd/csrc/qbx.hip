@@ -2,14 +2,16 @@
 //
 // Kernels
 //   qbx_prep         one workgroup per problem: bit-ordered W and d tables, the inner-block
-//                    step constants delta_t, the energy bound (int32 admission / FP32
-//                    filter margin), and reset of the per-problem reduction state.
+//                    step constants delta_t, the energy bound (int32 admission), for float
+//                    problems the quantized int32 copies of the tables the filter walks,
+//                    and reset of the per-problem reduction state.
 //                    Replaces the reference's cache building (qbf.py:246-289,
 //                    qbf1.py:105-118), which there costs GEMMs over 2^B x B state tables.
 //   qbx_walk_i32<M>  persistent walk over a queue of 64-lane tiles; exact int32 energies
 //                    (v_add3_u32 / v_min3_i32 inner loop).  Replaces the hot loop
 //                    qbf.py:188-230 and the QB_F1 kernel qbf1.py:179-249.
-//   qbx_walk_f64<M>  same walk with an FP32 block filter and FP64 exact re-walks.
+//   qbx_walk_f64<M>  float problems: the same int32 walk over quantized tables as a filter
+//                    with a proven error bound, FP64 exact re-walks of the blocks it passes.
 //
 // Reduction of a problem: each tile writes its (E, rank) slot, takes an atomicMin on an
 // order-preserving 64-bit key of E and, when its key is not above the previous minimum,
@@ -123,11 +125,11 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     for (int i = 0; i < count; ++i) {
         Plan& p = plans[i];
         p.off_W = off;
-        off = align256(off + lay.esize * (size_t)p.n * LDW);
+        off = align256(off + table_W_bytes(kind, p.n));
         p.off_d = off;
-        off = align256(off + lay.esize * 64);
+        off = align256(off + table_d_bytes(kind));
         p.off_delta = off;
-        off = align256(off + (kind == KIND_I32 ? 4 : 12) * (size_t)(1u << p.M));
+        off = align256(off + table_delta_bytes(kind, p.M));
     }
     lay.off_red = off;
     off = align256(off + sizeof(RedDev) * count);

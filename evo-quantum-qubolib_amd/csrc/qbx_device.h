@@ -37,17 +37,27 @@ struct InstDev {
     uint32_t chunks;         // ceil(tiles / tpc); reduction slots are per chunk
     uint32_t pad0;
     const void* Q;           // source Q: int32 or double, row-major, variable order
-    void* W;                 // n x LDW, bit order
-    void* d;                 // n
-    void* delta;             // int32[2^M]  |  double[2^M] followed by float[2^M]
+    void* W;                 // n x LDW, bit order: int32 | double followed by int32 Wq
+    void* d;                 // 64 entries: int32 | double followed by int32 dq
+    void* delta;             // 2^M entries: int32 | double followed by int32 dlq
     RedDev* red;
     uint32_t* cand;          // chunks entries
     SlotDev* slots;          // chunks entries
     ResultDev* result;
-    double margin;           // FP32 filter margin (float kind)
-    int64_t bound;           // sum |d| + sum_{a<b} |W|  (int kind)
+    double qscale;           // float kind: quantization scale 2^s of Wq, dq, dlq
+    double qmargin;          // float kind: filter margin in quantized units
+    int64_t bound;           // sum |d| + sum_{a<b} |W|  (int kind; quantized for float)
     double fbound;           // same in double (float kind)
 };
+
+// Table regions of one problem (bytes), shared by the batch and serving layouts.
+__host__ __device__ inline size_t table_W_bytes(int kind, int n) {
+    return (size_t)n * LDW * (kind == KIND_I32 ? 4 : 12);
+}
+__host__ __device__ inline size_t table_d_bytes(int kind) { return 64 * (kind == KIND_I32 ? 4 : 12); }
+__host__ __device__ inline size_t table_delta_bytes(int kind, int M) {
+    return (size_t)(1u << M) * (kind == KIND_I32 ? 4 : 12);
+}
 
 struct TileRange {
     uint32_t inst;
@@ -106,39 +116,69 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             I.status = red_i[0] > 0x7ffffffeLL ? QBX_ERANGE : 0;
         }
     } else {
+        // FP64 tables for the exact re-walks, then their quantized int32 copies for the filter
         const double* Q = (const double*)I.Q;
         double* W = (double*)I.W;
         double* d = (double*)I.d;
         double* dld = (double*)I.delta;
-        float* dls = (float*)(dld + (1 << M));
+        int32_t* Wq = (int32_t*)(W + (size_t)n * LDW);
+        int32_t* dq = (int32_t*)(d + 64);
+        int32_t* dlq = (int32_t*)(dld + (1 << M));
         double acc = 0.0;
+        bool finite = true;
         for (int idx = tid; idx < n * LDW; idx += blockDim.x) {
             const int a = idx / LDW, b = idx - a * LDW;
             const double w = w_elem<double, double>(Q, n, a, b);
             W[idx] = w;
+            finite = finite && (w - w == 0.0);
             if (b > a && b < n) acc += fabs(w);
         }
-        for (int a = tid; a < n; a += blockDim.x) {
-            const double v = d_elem<double, double>(Q, n, a);
+        for (int a = tid; a < 64; a += blockDim.x) {
+            const double v = a < n ? d_elem<double, double>(Q, n, a) : 0.0;
             d[a] = v;
+            finite = finite && (v - v == 0.0);
             acc += fabs(v);
         }
-        for (uint32_t t = tid; t < (1u << M); t += blockDim.x) {
-            const double v = delta_elem<double, double>(Q, n, M, t);
-            dld[t] = v;
-            dls[t] = (float)v;
-        }
-        red_f[tid] = acc;
+        for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
+            dld[t] = delta_elem<double, double>(Q, n, M, t);
+        red_f[tid] = finite ? acc : __builtin_inf();
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
             if (tid < s) red_f[tid] += red_f[tid + s];
             __syncthreads();
         }
+        const double fb = red_f[0];
+        const bool ok = fb <= 1.0e300;
+        const double scale = ok ? q_scale(fb, n) : 1.0;
+        __syncthreads();
+        long long qacc = 0;
+        for (int idx = tid; idx < n * LDW; idx += blockDim.x) {
+            const int a = idx / LDW, b = idx - a * LDW;
+            const int32_t q = ok ? (int32_t)rint(W[idx] * scale) : 0;
+            Wq[idx] = q;
+            if (b > a && b < n) qacc += q < 0 ? -(long long)q : q;
+        }
+        for (int a = tid; a < 64; a += blockDim.x) {
+            const int32_t q = ok ? (int32_t)rint(d[a] * scale) : 0;
+            dq[a] = q;
+            qacc += q < 0 ? -(long long)q : q;
+        }
+        red_i[tid] = qacc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) red_i[tid] += red_i[tid + s];
+            __syncthreads();
+        }
+        // step constants of the filter from the quantized couplings (exact int32 sums)
+        for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
+            dlq[t] = (int32_t)delta_from_table<int32_t, long long>(Wq, M, t);
         if (tid == 0) {
-            I.fbound = red_f[0];
-            I.margin = filter_margin(red_f[0], M);
-            // the FP32 filter needs finite values well inside float range
-            I.status = (red_f[0] <= 1.0e30) ? 0 : QBX_ERANGE;
+            I.fbound = fb;
+            I.qscale = scale;
+            I.qmargin = q_margin(n);
+            I.bound = red_i[0];
+            // finite energies whose quantized bound fits int32 (q_scale guarantees it)
+            I.status = (ok && red_i[0] <= 0x7ffffffeLL) ? 0 : QBX_ERANGE;
         }
     }
     if (tid == 0) {
@@ -328,23 +368,30 @@ static __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restr
         uint64_t rank = ~0ull;
         if (I.status == 0) {
             const int n = I.n, L = I.L, H = n - L;
+            const double* W = (const double*)I.W;
+            const double* d = (const double*)I.d;
             const double* dld = (const double*)I.delta;
-            const float* dsrc = (const float*)(dld + (1 << M));
-            float dls[1 << M];
+            const int32_t* Wq = (const int32_t*)(W + (size_t)n * LDW);
+            const int32_t* dq = (const int32_t*)(d + 64);
+            const int32_t* dsrc = (const int32_t*)(dld + (1 << M));
+            int32_t dlq[1 << M];
 #pragma unroll
-            for (int k = 0; k < (1 << M); ++k) dls[k] = dsrc[k + (lane & zero)];
+            for (int k = 0; k < (1 << M); ++k) dlq[k] = dsrc[k + (lane & zero)];
             const uint32_t t0 = chunk * I.tpc;
             const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
             // branch and bound across waves through the problem's running minimum key
             WaveBound wb{&I.red->minkey, __builtin_inf()};
+            // per-lane candidate records, lane-interleaved (entry k of lane l at k * 64 + l)
+            __shared__ uint32_t qc[QCAP * 64];
+            __shared__ int32_t qmn[QCAP * 64];
             for (uint32_t tile = t0; tile < t1; ++tile) {
                 // every lane walks (the walk shuffles across the wave); a lane past the end
                 // of a small problem repeats lane 0's work and its result is dropped
                 const uint64_t lam0 = (uint64_t)tile * 64u;
                 const uint64_t lam = lam0 + (uint64_t)lane;
                 const bool active = H >= 6 || lam < (1ull << H);
-                const LaneOutF64 o = walk_lane_f64<M, OMAX>((const double*)I.W, (const double*)I.d, n, L, I.B,
-                                                            active ? lam : lam0, dls, dld, I.margin, wb);
+                const LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq, dq, W, d, n, L, I.B, active ? lam : lam0, dlq, dld,
+                                                            I.qscale, I.qmargin, wb, qc + lane, qmn + lane, 64);
                 if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
             }
         }

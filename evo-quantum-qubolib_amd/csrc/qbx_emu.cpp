@@ -7,10 +7,16 @@
 // every (n, L, B, M) shape before any GPU time is spent.
 #include <stdint.h>
 #include <string.h>
+#include <math.h>
 #include <vector>
+#define QBX_EMU_STATS 1
 #include "qbx_walk.h"
 
 using namespace qbx;
+
+namespace qbx {
+unsigned long long emu_stat_flushes = 0, emu_stat_rewalks = 0;
+}
 
 namespace {
 
@@ -46,30 +52,47 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
 template <int M, int OMAX>
 int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_out) {
     if (L < M || L > M + OMAX || L > n) return -2;
-    std::vector<double> W((size_t)n * LDW, 0.0), d(n, 0.0), dld(1 << M, 0.0);
-    std::vector<float> dls(1 << M, 0.0f);
+    // the tables qbx_prep builds: FP64 W, d, delta, and their quantized int32 copies
+    std::vector<double> W((size_t)n * LDW, 0.0), d(64, 0.0), dld(1 << M, 0.0);
+    std::vector<int32_t> Wq((size_t)n * LDW, 0), dq(64, 0), dlq(1 << M, 0);
     double bound = 0.0;
     for (int a = 0; a < n; ++a) {
         d[a] = d_elem<double, double>(Q, n, a);
-        bound += d[a] < 0 ? -d[a] : d[a];
+        bound += fabs(d[a]);
         for (int b = 0; b < LDW; ++b) {
             W[(size_t)a * LDW + b] = w_elem<double, double>(Q, n, a, b);
-            if (b > a && b < n) bound += W[(size_t)a * LDW + b] < 0 ? -W[(size_t)a * LDW + b] : W[(size_t)a * LDW + b];
+            if (b > a && b < n) bound += fabs(W[(size_t)a * LDW + b]);
         }
     }
+    if (!(bound <= 1.0e300)) return -4;
+    const double scale = q_scale(bound, n);
+    long long qbound = 0;
+    for (int a = 0; a < n; ++a) {
+        dq[a] = (int32_t)rint(d[a] * scale);
+        qbound += dq[a] < 0 ? -(long long)dq[a] : dq[a];
+        for (int b = 0; b < LDW; ++b) {
+            const int32_t q = (int32_t)rint(W[(size_t)a * LDW + b] * scale);
+            Wq[(size_t)a * LDW + b] = q;
+            if (b > a && b < n) qbound += q < 0 ? -(long long)q : q;
+        }
+    }
+    if (qbound > 0x7ffffffeLL) return -4;
     for (uint32_t t = 0; t < (1u << M); ++t) {
         dld[t] = delta_elem<double, double>(Q, n, M, t);
-        dls[t] = (float)dld[t];
+        dlq[t] = (int32_t)delta_from_table<int32_t, long long>(Wq.data(), M, t);
     }
-    const double margin = filter_margin(bound, M);
+    const double qm = q_margin(n);
     const int H = n - L;
     const uint64_t lanes = 1ull << H;
     double bE = __builtin_inf();
     uint64_t bR = ~0ull;
-    // lanes see the best energy found so far as their bound, as the device's waves do
+    // one bound shared by all lanes, as the device's waves share the problem's key
+    HostBound hb{__builtin_inf()};
+    uint32_t buf_c[QCAP];
+    int32_t buf_m[QCAP];
     for (uint64_t lam = 0; lam < lanes; ++lam) {
-        FixedBound fb{bE};
-        LaneOutF64 o = walk_lane_f64<M, OMAX>(W.data(), d.data(), n, L, B, lam, dls.data(), dld.data(), margin, fb);
+        LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq.data(), dq.data(), W.data(), d.data(), n, L, B, lam, dlq.data(),
+                                              dld.data(), scale, qm, hb, buf_c, buf_m, 1);
         if (o.E < bE || (o.E == bE && o.rank < bR)) { bE = o.E; bR = o.rank; }
     }
     *E_out = bE;
@@ -81,7 +104,8 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
 
 extern "C" {
 
-// Returns 0 on success; -1 unsupported M; -2 unsupported L.
+// Returns 0 on success; -1 unsupported M; -2 unsupported L; -3 the two i32 flows disagree;
+// -4 float energies out of range.
 int qbx_emu_solve_i32(const int32_t* Q, int n, int B, int L, int M, int64_t* E, uint64_t* rank,
                       uint64_t* state) {
     int rc;
@@ -103,10 +127,18 @@ int qbx_emu_solve_f64(const double* Q, int n, int B, int L, int M, double* E, ui
         case 1: rc = emu_f64<1, 19>(Q, n, B, L, E, rank); break;
         case 4: rc = emu_f64<4, 16>(Q, n, B, L, E, rank); break;
         case 6: rc = emu_f64<6, 14>(Q, n, B, L, E, rank); break;
+        case 7: rc = emu_f64<7, 13>(Q, n, B, L, E, rank); break;
         default: return -1;
     }
     if (rc == 0) *state = rank_to_state(*rank, B);
     return rc;
+}
+
+// float filter path counters since the last call (buffer overflows, FP64 block re-walks)
+void qbx_emu_take_stats(uint64_t* flushes, uint64_t* rewalks) {
+    *flushes = emu_stat_flushes;
+    *rewalks = emu_stat_rewalks;
+    emu_stat_flushes = emu_stat_rewalks = 0;
 }
 
 uint64_t qbx_emu_state_to_rank(uint64_t s, int B) { return state_to_rank(s, B); }

@@ -34,6 +34,7 @@ static const Variant kVariants[] = {
     {KIND_I32, 6, 14, (const void*)qbx_walk_i32<6, 14>},
     {KIND_I32, 4, 16, (const void*)qbx_walk_i32<4, 16>},
     {KIND_I32, 1, 19, (const void*)qbx_walk_i32<1, 19>},
+    {KIND_F64, 7, 13, (const void*)qbx_walk_f64<7, 13>},
     {KIND_F64, 6, 14, (const void*)qbx_walk_f64<6, 14>},
     {KIND_F64, 4, 16, (const void*)qbx_walk_f64<4, 16>},
     {KIND_F64, 1, 19, (const void*)qbx_walk_f64<1, 19>},
@@ -100,7 +101,7 @@ static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mo
     p.kind = kind;
     p.B = (o.tie_B < 0) ? (n < 20 ? n : 20) : o.tie_B;
     if (p.B > n) return set_err(QBX_EINVAL, "problem %d: tie_B=%d exceeds n=%d", index, p.B, n);
-    const int wantM = (o.inner_bits < 0) ? (kind == KIND_I32 ? 7 : 6) : o.inner_bits;
+    const int wantM = (o.inner_bits < 0) ? 7 : o.inner_bits;
     p.variant = pick_variant(kind, wantM, n);
     if (p.variant < 0) return set_err(QBX_EINVAL, "problem %d: no kernel for inner_bits=%d", index, wantM);
     p.M = kVariants[p.variant].M;

@@ -405,9 +405,9 @@ int server_alloc(qbx_server* S) {
         }
     SlotLayout& L = S->L;
     size_t off = 0;
-    L.off_W = off; off = align256(off + (size_t)max_n * LDW * 8);
-    L.off_d = off; off = align256(off + 64 * 8);
-    L.off_delta = off; off = align256(off + 128 * 12);
+    L.off_W = off; off = align256(off + table_W_bytes(KIND_F64, max_n));
+    L.off_d = off; off = align256(off + table_d_bytes(KIND_F64));
+    L.off_delta = off; off = align256(off + table_delta_bytes(KIND_F64, 7));
     L.off_red = off; off = align256(off + sizeof(RedDev));
     L.off_cand = off; off = align256(off + 4 * (size_t)max_chunks);
     L.off_slots = off; off = align256(off + sizeof(SlotDev) * (size_t)max_chunks);

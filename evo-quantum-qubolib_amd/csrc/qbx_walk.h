@@ -31,6 +31,8 @@
 //     beats the lane's best in (E, rank) order -- rare after the first few blocks.
 //   * ordering: block_key() gives rank bits [M, L) of a block's states and rank_lo() the
 //     inner bits [0, M), so (E, lane, key, rank_lo) orders states as (E, rank) does.
+//   * float Q: the same int32 walk over quantized tables is a filter with a proven error
+//     bound; blocks that pass it are rebuilt and walked exactly in FP64 (walk_lane_q32).
 #pragma once
 #include <stdint.h>
 #include <utility>
@@ -40,11 +42,21 @@
 #define QBX_HD __host__ __device__ __forceinline__
 #define QBX_HD_MEMBER __host__ __device__ __forceinline__
 #else
+#include <math.h>
 #define QBX_HD static inline
 #define QBX_HD_MEMBER inline
 #endif
 
 namespace qbx {
+
+// Path counters of the host build (the CPU tests check that every path of the float
+// filter is exercised); compiled out of the device build.
+#ifdef QBX_EMU_STATS
+extern unsigned long long emu_stat_flushes, emu_stat_rewalks;
+#define QBX_STAT(x) (x)
+#else
+#define QBX_STAT(x) ((void)0)
+#endif
 
 // ---------------------------------------------------------------------------- bits
 QBX_HD uint64_t shr64(uint64_t v, int b) { return b >= 64 ? 0ull : (v >> b); }
@@ -147,9 +159,6 @@ struct LaneOutI32 {
 QBX_HD int32_t min3_i32(int32_t mn, int32_t a, int32_t b) {
     const int32_t m = mn < a ? mn : a;
     return m < b ? m : b;
-}
-QBX_HD float min3_f32(float mn, float a, float b) {
-    return __builtin_fminf(__builtin_fminf(mn, a), b);
 }
 // Empty volatile asm: makes a value opaque to the optimiser (no instruction is emitted).
 // Used only on the rare re-walk path, so that its energies are not merged with the
@@ -448,149 +457,55 @@ QBX_HD LaneOutI32 walk_lane_i32(const int32_t* __restrict__ W, const int32_t* __
     return out;
 }
 
-// ------------------------------------------------------- float-filter / FP64 walk
-// Slow state in FP64, block walk in FP32 as a filter; any block whose FP32 minimum is
-// within `margin` of the lane's FP64 best is re-walked in FP64 and compared exactly.
-template <int M, int OMAX>
-struct LaneF64 {
-    double E0;
-    double f[M];
-    double u[OMAX];
-};
-
+// ----------------------------------------------- float problems: int32 filter, FP64 walk
+// The block filter runs on quantized tables Wq = rint(W 2^s), dq = rint(d 2^s) in exact
+// int32 arithmetic (the int walk above, unchanged).  Each of the terms d_a, W_ab that a
+// state x switches on is off by at most 1/2 after quantization, so with T(x) <= n(n+1)/2
+// terms, |Eq(x) - 2^s E(x)| <= n(n+1)/4.  A block can therefore hold a state with
+// E(x) <= b only if its int32 minimum is <= 2^s b + n(n+1)/4; the blocks that pass are
+// rebuilt from scratch in FP64 and re-walked exactly.
 struct LaneOutF64 {
     double E;
     uint64_t rank;
 };
 
-template <int M>
-QBX_HD float block_min_f32(float E0, const float* f, const float* dl) {
-    float E = E0, mn = E0, odd = E0;
-#pragma unroll
-    for (int t = 1; t < (1 << M); ++t) {
-        const int k = __builtin_ctz((unsigned)t);
-        const bool up = ((((unsigned)t ^ ((unsigned)t >> 1)) >> k) & 1u) != 0u;
-        E = up ? (E + f[k]) : (E - f[k]);
-        E = E + dl[t];
-        if (t & 1) odd = E;
-        else mn = min3_f32(mn, odd, E);
-    }
-    return min3_f32(mn, odd, odd);
+// 2^s: the largest power of two that keeps the quantized energy bound within int32
+// (sum |rint(v 2^s)| <= bound 2^s + (number of terms) / 2 <= 2^31 - 2).
+QBX_HD double q_scale(double bound, int n) {
+    const double half_terms = 0.25 * (double)n * (double)(n + 1);
+    const double room = 2147483646.0 - half_terms - 1.0;
+    if (!(bound > 0.0)) return 1.0;
+    const double r = room / bound;
+    if (!(r < 0x1p1000)) return 0x1p1000;   // tiny entries: any large scale is exact enough
+    int e = 0;
+    (void)frexp(r, &e);                     // r = m 2^e, m in [0.5, 1)
+    return ldexp(1.0, e - 1);
+}
+// margin in quantized units: n(n+1)/4, plus one for the FP64 rounding of the energies
+QBX_HD double q_margin(int n) { return 0.25 * (double)n * (double)(n + 1) + 1.0; }
+// int32 threshold for "can hold a state with E <= b"
+QBX_HD int32_t q_threshold(double b, double scale, double qm) {
+    if (!(b < 1.0e300)) return 0x7fffffff;
+    const double t = floor(b * scale + qm);
+    if (t >= 2147483647.0) return 0x7fffffff;
+    if (t <= -2147483648.0) return (int32_t)0x80000000;
+    return (int32_t)t;
+}
+// inner-step constant delta_t from a bit-ordered table (row k, column l = W[k][l])
+template <class T, class Acc>
+QBX_HD Acc delta_from_table(const T* W, int M, uint32_t t) {
+    if (t == 0 || t >= (1u << M)) return Acc(0);
+    const int k = ctz32(t);
+    const uint32_t g = t ^ (t >> 1);
+    const uint32_t y = (t - 1) ^ ((t - 1) >> 1);
+    const bool up = ((g >> k) & 1u) != 0u;
+    Acc s = Acc(0);
+    for (int l = 0; l < M; ++l)
+        if (l != k && ((y >> l) & 1u)) s += Acc(W[k * LDW + l]);
+    return up ? s : Acc(0) - s;
 }
 
-template <int M, int OMAX, int I>
-QBX_HD void trans_static_f64(LaneF64<M, OMAX>& s, const double* __restrict__ WJ, bool up) {
-    constexpr int J = M + I;
-    const double field = s.u[I] + (I > 0 ? WJ[J - 1] : 0.0);
-    if (up) {
-        s.E0 += field;
-#pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] += WJ[k];
-#pragma unroll
-        for (int q = 0; q < I; ++q) s.u[q] += WJ[M + q];
-    } else {
-        s.E0 -= field;
-#pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] -= WJ[k];
-#pragma unroll
-        for (int q = 0; q < I; ++q) s.u[q] -= WJ[M + q];
-    }
-}
-
-template <int M, int OMAX>
-QBX_HD void trans_dyn_f64(LaneF64<M, OMAX>& s, const double* __restrict__ WJ, int i, bool up) {
-    const int J = M + i;
-    double uJ = 0.0;
-#pragma unroll
-    for (int q = 0; q < OMAX; ++q) if (q == i) uJ = s.u[q];
-    const double field = uJ + WJ[J - 1];
-    if (up) {
-        s.E0 += field;
-#pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] += WJ[k];
-#pragma unroll
-        for (int q = 0; q < OMAX; ++q) if (q < i) s.u[q] += WJ[M + q];
-    } else {
-        s.E0 -= field;
-#pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] -= WJ[k];
-#pragma unroll
-        for (int q = 0; q < OMAX; ++q) if (q < i) s.u[q] -= WJ[M + q];
-    }
-}
-
-// f32 value >= x (x finite or +inf): x rounded to float, bumped up one ulp if it fell below.
-QBX_HD float f32_next_up(float r) {
-    union { float f; uint32_t u; } v;
-    v.f = r;
-    if (r != r || v.u == 0x7f800000u) return r;
-    if (v.u == 0x80000000u) { v.u = 1u; return v.f; }
-    if (v.u & 0x80000000u) v.u -= 1u; else v.u += 1u;
-    return v.f;
-}
-QBX_HD float f32_at_least(double x) {
-    float r = (float)x;
-    if ((double)r < x) r = f32_next_up(r);
-    return r;
-}
-
-// Lane set-up in FP64 (same split as lane_setup_i32: high lane bits are wave-uniform).
-template <int M, int OMAX>
-QBX_HD void lane_setup_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const double* __restrict__ d,
-                           int n, int L, int B, uint64_t lam) {
-    const int H = n - L;
-    const int O = L - M;
-    const int Hlo = H < 6 ? H : 6;
-    const uint64_t p = lane_prefix(lam, L, B);
-    const uint64_t p_hi = shr64(lane_prefix(lam & ~63ull, L, B), 6);
-    const uint32_t p_lo = (uint32_t)(p & 63ull);
-    double Ec = 0.0, fc[M], uc[OMAX], lc[6];
-#pragma unroll
-    for (int k = 0; k < M; ++k) fc[k] = d[k];
-#pragma unroll
-    for (int q = 0; q < OMAX; ++q) uc[q] = (q < O) ? d[M + q] : 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) lc[i] = (i < Hlo) ? d[L + i] : 0.0;
-    for (int j = Hlo; j < H; ++j) {
-        if (!((p_hi >> (j - Hlo)) & 1ull)) continue;
-        const int a = L + j;
-        const double* Wa = W + a * LDW;
-        double e = d[a];
-        for (int j2 = j + 1; j2 < H; ++j2)
-            if ((p_hi >> (j2 - Hlo)) & 1ull) e += Wa[L + j2];
-        Ec += e;
-#pragma unroll
-        for (int k = 0; k < M; ++k) fc[k] += Wa[k];
-#pragma unroll
-        for (int q = 0; q < OMAX; ++q) uc[q] += Wa[M + q];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) lc[i] += (i < Hlo) ? Wa[L + i] : 0.0;
-    }
-    double E0 = Ec;
-#pragma unroll
-    for (int k = 0; k < M; ++k) s.f[k] = fc[k];
-#pragma unroll
-    for (int q = 0; q < OMAX; ++q) s.u[q] = uc[q];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        if (i >= Hlo) break;
-        if (!((p_lo >> i) & 1u)) continue;
-        const int a = L + i;
-        const double* Wa = W + a * LDW;
-        double e = lc[i];
-#pragma unroll
-        for (int i2 = i + 1; i2 < 6; ++i2)
-            if (i2 < Hlo && ((p_lo >> i2) & 1u)) e += Wa[L + i2];
-        E0 += e;
-#pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] += Wa[k];
-#pragma unroll
-        for (int q = 0; q < OMAX; ++q) s.u[q] += Wa[M + q];
-    }
-    s.E0 = E0;
-}
-
-// E0 and inner fields of an arbitrary block from scratch (hs: bits above the block).
+// E0 and inner fields of an arbitrary block, from scratch, in FP64 (hs: bits above the block).
 template <int M>
 QBX_HD void block_state_f64(const double* __restrict__ W, const double* __restrict__ d, int n,
                             uint64_t hs, double& E0, double* f) {
@@ -634,107 +549,201 @@ QBX_HD void block_rewalk_f64(double E0, const double* f, const double* __restric
     }
 }
 
-// One block of an unrolled float group: static transition, FP32 block minimum from the FP64
-// slow state, candidate bit when the minimum is within the lane's threshold.
+// Rebuild block c of the lane (prefix p, O outer bits) in FP64 and walk it exactly.
+template <int M>
+QBX_HD void block_exact_f64(const double* __restrict__ W, const double* __restrict__ d, int n, uint64_t p,
+                            int O, uint32_t c, uint64_t laneRank, int SBk, uint32_t pmaskS, int B, int parp,
+                            const double* __restrict__ dld, double& bestE, uint64_t& bestR) {
+    const uint32_t gc = c ^ (c >> 1);
+    double bE0, bf[M];
+    block_state_f64<M>(W, d, n, (p << O) | (uint64_t)gc, bE0, bf);
+    block_rewalk_f64<M>(bE0, bf, dld, laneRank | ((uint64_t)block_key(c, SBk, pmaskS) << M), B,
+                        parp ^ parity32(gc), bestE, bestR);
+}
+
+// Deferred candidate blocks of one lane: (block index, int32 block minimum) records in a
+// per-lane buffer (LDS on the device, entry k of a lane at k * stride).  A block that
+// passes the filter is only recorded; it is rebuilt and walked in FP64 at the end of the
+// lane, or at a group boundary when the buffer might not hold another group, and only if
+// it still passes then (the lane minimum and the shared bound keep tightening, so most
+// records never are).  A group adds at most 2^GROUP_BITS records, so nothing is walked
+// in FP64 inside the unrolled group.
+constexpr int QCAP = 32;
+constexpr uint32_t QSYNC = 8;   // groups between exchanges of the shared bound
+
+struct QBuf {
+    uint32_t* c;
+    int32_t* m;
+    int stride;
+    int count;
+};
+
+// Lane state of the filtered walk besides the int32 block state.
+struct QLane {
+    int32_t laneMin;      // least int32 block minimum seen by the lane
+    int32_t thrB;         // threshold from the exact / shared bound
+    int32_t thr;          // min(thrB, laneMin + 2 qm): the filter
+    int32_t two_qm;       // 2 * n(n+1)/4 + 2, in quantized units
+};
+
+QBX_HD int32_t sat_add_i32(int32_t a, int32_t b) {     // b >= 0
+    return a > 0x7fffffff - b ? 0x7fffffff : a + b;
+}
+QBX_HD void q_update_thr(QLane& q) {
+    const int32_t loc = sat_add_i32(q.laneMin, q.two_qm);
+    q.thr = q.thrB < loc ? q.thrB : loc;
+}
+
+template <int M>
+struct QCtx {    // what a flush needs to rebuild and walk a block exactly
+    const double* W;
+    const double* d;
+    const double* dld;
+    int n, O, SBk, B, parp;
+    uint64_t p, laneRank;
+    uint32_t pmaskS;
+};
+
+// Drop the records the filter no longer passes; with `walk`, walk the rest in FP64 and
+// empty the buffer.
+template <int M>
+QBX_HD void q_compact(QBuf& b, const QLane& q, const QCtx<M>& x, bool walk, double& bestE, uint64_t& bestR) {
+    int keep = 0;
+    for (int k = 0; k < b.count; ++k) {
+        const int32_t m = b.m[k * b.stride];
+        if (m > q.thr) continue;
+        const uint32_t c = b.c[k * b.stride];
+        if (walk) {
+            QBX_STAT(++emu_stat_rewalks);
+            block_exact_f64<M>(x.W, x.d, x.n, x.p, x.O, c, x.laneRank, x.SBk, x.pmaskS, x.B, x.parp, x.dld,
+                               bestE, bestR);
+        } else {
+            b.c[keep * b.stride] = c;
+            b.m[keep * b.stride] = m;
+            ++keep;
+        }
+    }
+    b.count = keep;
+}
+
+// One block of an unrolled group of the filtered walk: static transition of the int32
+// state, int32 block minimum, lane minimum and filter update, candidate record.
 template <int M, int OMAX, int J>
-QBX_HD bool group_step_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const float* dls,
-                           uint32_t c0, uint32_t nblk, float thr, uint32_t& cand) {
+QBX_HD bool group_step_q32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ Wq, const int32_t* dlq,
+                           uint32_t c0, uint32_t nblk, QLane& q, QBuf& b) {
     const uint32_t c = c0 + (uint32_t)J;
     if (J > 0) {
         if (c >= nblk) return false;
         constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
         const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
                                              : (((c0 >> GROUP_BITS) & 1u) == 0u);
-        trans_static_f64<M, OMAX, i>(s, W + (M + i) * LDW, up);
+        trans_static_i32<M, OMAX, i>(s, Wq + (M + i) * LDW, up);
     }
-    float fs[M];
-#pragma unroll
-    for (int k = 0; k < M; ++k) fs[k] = (float)s.f[k];
-    const float mn = block_min_f32<M>((float)s.E0, fs, dls);
-    if (mn <= thr) cand |= 1u << J;
+    const int32_t mn = block_min_i32<M>(s.E0, s.f, s.nf, dlq);
+    if (mn <= q.thr) {           // rare after the first blocks: record the block
+        q.laneMin = mn < q.laneMin ? mn : q.laneMin;
+        q_update_thr(q);
+        b.c[b.count * b.stride] = c;
+        b.m[b.count * b.stride] = mn;
+        ++b.count;
+    }
     return true;
 }
 
 template <int M, int OMAX, int... Js>
-QBX_HD void group_f64(LaneF64<M, OMAX>& s, const double* __restrict__ W, const float* dls,
-                      uint32_t c0, uint32_t nblk, float thr, uint32_t& cand,
-                      std::integer_sequence<int, Js...>) {
-    (void)(group_step_f64<M, OMAX, Js>(s, W, dls, c0, nblk, thr, cand) && ...);
+QBX_HD void group_q32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ Wq, const int32_t* dlq,
+                      uint32_t c0, uint32_t nblk, QLane& q, QBuf& b, std::integer_sequence<int, Js...>) {
+    (void)(group_step_q32<M, OMAX, Js>(s, Wq, dlq, c0, nblk, q, b) && ...);
 }
 
-// dls: FP32 inner deltas (registers); dld: FP64 inner deltas (memory, read on re-walks).
-// Source of an upper bound on the answer: refresh() returns an exact energy already reached
-// by some state of the problem (+inf if none); publish(e) offers the lane's exact best.
-// Blocks whose FP32 minimum exceeds min(lane best, bound) + margin cannot hold the answer.
-// The host build uses FixedBound; the device shares a running bound across waves.
-struct FixedBound {
+// Source of an upper bound on the answer shared by the lanes of one problem: refresh()
+// returns an upper bound on the energy of some state (+inf if none yet); publish(e)
+// offers one.  The host build keeps it in a plain double shared by the lanes it walks in
+// turn; the device shares it across waves through the problem's reduction key.
+struct HostBound {
     double b;
     QBX_HD_MEMBER double refresh() const { return b; }
-    QBX_HD_MEMBER void publish(double) {}
+    QBX_HD_MEMBER void publish(double e) { b = e < b ? e : b; }
 };
 
+// Walk one lane of a float problem: int32 filter over quantized tables (Wq, dq, dlq in
+// registers), FP64 exact re-walk (W, d, dld) of the blocks that still pass at the end.
+// Returns the lane's exact (E, rank) minimum over the blocks it re-walked, which include
+// every block that can hold a state of energy <= the problem's final bound (+inf, ~0 if
+// it re-walked none).
+//
+// Why the filter is safe: with qm0 = n(n+1)/4, |Eq(x) - s E(x)| <= qm0 for every state x.
+//   * shared bound b (an upper bound on some state's energy, so >= the answer E*): a
+//     block holding a state x with E(x) <= b has int32 minimum <= s b + qm0 = thrB;
+//   * lane minimum mq = Eq(xq): a state x of the lane with E(x) <= E(xq) has
+//     Eq(x) <= s E(x) + qm0 <= s E(xq) + qm0 <= mq + 2 qm0.
+// Both thresholds only decrease during the walk, so a block dropped at any time fails the
+// final filter too.  The lane publishes (mq + qm0 + 1) / s, an upper bound on E(xq).
 template <int M, int OMAX, class Bound>
-QBX_HD LaneOutF64 walk_lane_f64(const double* __restrict__ W, const double* __restrict__ d,
-                                int n, int L, int B, uint64_t lam, const float* dls,
-                                const double* __restrict__ dld, double margin, Bound& src) {
-    double bound = src.refresh();
+QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
+                                const double* __restrict__ W, const double* __restrict__ d,
+                                int n, int L, int B, uint64_t lam, const int32_t* dlq,
+                                const double* __restrict__ dld, double scale, double qm, Bound& src,
+                                uint32_t* buf_c, int32_t* buf_m, int buf_stride) {
     static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
     const int O = L - M;
     const uint64_t p = lane_prefix(lam, L, B);
-    LaneF64<M, OMAX> s;
-    lane_setup_f64<M, OMAX>(s, W, d, n, L, B, lam);
+    LaneI32<M, OMAX> s;
+    lane_setup_i32<M, OMAX>(s, Wq, dq, n, L, B, lam);
 
-    const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
-    const int parp = parity64(p);
-    const uint32_t pmaskS = (parp && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
-    const uint64_t laneRank = shl64(lam, L);
+    QCtx<M> x;
+    x.W = W; x.d = d; x.dld = dld; x.n = n; x.O = O; x.B = B; x.p = p;
+    x.SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
+    x.parp = parity64(p);
+    x.pmaskS = (x.parp && O > x.SBk) ? (lowmask32(O - x.SBk) << x.SBk) : 0u;
+    x.laneRank = shl64(lam, L);
 
-    // the lane's first block is walked exactly at once, so the filter starts tight
+    double bound = src.refresh();
     double bestE = __builtin_inf();
     uint64_t bestR = ~0ull;
-    block_rewalk_f64<M>(s.E0, s.f, dld, laneRank | ((uint64_t)block_key(0u, SBk, pmaskS) << M), B, parp,
-                        bestE, bestR);
-    float thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
+    QLane q;
+    q.laneMin = 0x7fffffff;
+    q.two_qm = (int32_t)(2.0 * qm);
+    q.thrB = q_threshold(bound, scale, qm);
+    q_update_thr(q);
+    QBuf b{buf_c, buf_m, buf_stride, 0};
 
     const uint32_t nblk = 1u << O;
     for (uint32_t c0 = 0;;) {
-        uint32_t cand = 0u;
-        group_f64<M, OMAX>(s, W, dls, c0, nblk, thr, cand, std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
-        if (c0 == 0) cand &= ~1u;   // block 0 is already exact
-        while (cand) {               // rare: rebuild each candidate block and walk it in FP64
-            const uint32_t j = (uint32_t)__builtin_ctz(cand);
-            cand &= cand - 1u;
-            const uint32_t c = c0 + j;
-            const uint32_t gc = c ^ (c >> 1);
-            double bE0, bf[M];
-            block_state_f64<M>(W, d, n, (p << O) | (uint64_t)gc, bE0, bf);
-            block_rewalk_f64<M>(bE0, bf, dld, laneRank | ((uint64_t)block_key(c, SBk, pmaskS) << M), B,
-                                parp ^ parity32(gc), bestE, bestR);
-            thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
+        group_q32<M, OMAX>(s, Wq, dlq, c0, nblk, q, b, std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
+        if (b.count > QCAP - (1 << GROUP_BITS)) {   // room for one more group of records
+            q_compact<M>(b, q, x, false, bestE, bestR);
+            if (b.count > QCAP - (1 << GROUP_BITS)) {
+                QBX_STAT(++emu_stat_flushes);
+                q_compact<M>(b, q, x, true, bestE, bestR);
+            }
         }
-        // share progress with the rest of the problem: offer the lane's best, take the
-        // problem's running bound (both are exact energies of real states)
-        src.publish(bestE);
-        const double b2 = src.refresh();
-        bound = b2 < bound ? b2 : bound;
-        thr = f32_at_least((bestE < bound ? bestE : bound) + margin);
+        // share progress with the rest of the problem and take its running bound: after
+        // the first group (when it tightens the most), then every QSYNC groups
+        if ((c0 & ((QSYNC << GROUP_BITS) - 1u)) == 0u) {
+            double mine = bestE;
+            if (q.laneMin != 0x7fffffff) {
+                const double ub = ((double)q.laneMin + qm) / scale;     // qm = qm0 + 1
+                mine = ub < mine ? ub : mine;
+            }
+            src.publish(mine);
+            const double b2 = src.refresh();
+            bound = b2 < bound ? b2 : bound;
+            const double bb = bestE < bound ? bestE : bound;
+            q.thrB = q_threshold(bb, scale, qm);
+            q_update_thr(q);
+        }
         c0 += 1u << GROUP_BITS;
         if (c0 >= nblk) break;
         const int i = ctz32(c0);
         const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
-        trans_dyn_f64<M, OMAX>(s, W + (M + i) * LDW, i, up);
+        trans_dyn_i32<M, OMAX>(s, Wq + (M + i) * LDW, i, up);
     }
+    q_compact<M>(b, q, x, true, bestE, bestR);
     LaneOutF64 out;
     out.E = bestE;
     out.rank = bestR;
     return out;
-}
-
-// FP32 filter margin: every FP32 block value differs from its FP64 counterpart by at most
-// u * bound * (1 + 6 * 2^M), u = 2^-24 (one rounding for E0, f_k and delta_t each, and
-// two additions per step with partial sums bounded by 2 * bound).  8 * 2^M is used.
-QBX_HD double filter_margin(double bound, int M) {
-    return bound * 5.9604644775390625e-08 /* 2^-24 */ * 8.0 * (double)(1 << M);
 }
 
 // Order-preserving 64-bit keys for the per-instance atomic minimum.

@@ -8,8 +8,9 @@ Arithmetic path per problem:
   * integer Q with bound < 2^53: the FP64 walk, exact on integers (qbx_solve_f64).
   * float Q whose entries are all integers (MaxCut's default float32 matrices): the
       exact int32 walk, energies returned as floats.
-  * other float Q: the FP64 walk on the float64 values of Q (an FP32 block filter,
-      every candidate block re-walked in FP64).
+  * other float Q: the FP64 walk on the float64 values of Q (the int32 walk over tables
+      quantized to 2^s units filters blocks with a proven error bound; every block it
+      passes is re-walked in FP64).
   * anything else: OverflowError.  There is no CPU path: without libqbx.so or a GPU
     every call raises.
 

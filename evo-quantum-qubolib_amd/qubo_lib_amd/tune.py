@@ -133,9 +133,10 @@ def main(argv=None):
     res["seconds"] = round(time.time() - t0, 1)
     out = args.out or (_native.TUNING_DIR / f"{res['device']['name'].split(':')[0]}.json")
     out.parent.mkdir(parents=True, exist_ok=True)
-    if out.exists():
-        # merge: keep entries of kinds/modes not measured in this run
-        old = json.loads(out.read_text())
+    base = out if out.exists() else _native.TUNING_DIR / out.name
+    if base.exists():
+        # merge: keep entries of kinds/modes/n not measured in this run
+        old = json.loads(base.read_text())
         for key, table in res["walk_bits"].items():
             prev = old.get("walk_bits", {}).get(key)
             if prev is not None:

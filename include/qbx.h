@@ -33,7 +33,7 @@ extern "C" {
 #define QBX_EBUSY (-5)      /* server queue full / budget exhausted */
 
 #define QBX_KIND_I32 0      /* int32 matrices, exact int32 energies */
-#define QBX_KIND_F64 1      /* double matrices, FP32 filter + FP64 re-walk */
+#define QBX_KIND_F64 1      /* double matrices, quantized int32 filter + FP64 re-walk */
 #define QBX_MODE_SINGLE 0   /* one problem spread over the whole GPU */
 #define QBX_MODE_BATCH 1    /* many problems sharing the GPU (batches, serving) */
 
@@ -42,7 +42,7 @@ typedef struct qbx_ctx qbx_ctx_t;
 typedef struct {
     int32_t tie_B;        /* reference suffix width B for the tie rule; -1 = min(n, 20) */
     int32_t walk_bits;    /* bits walked by one lane (L); -1 = tuned default */
-    int32_t inner_bits;   /* unrolled inner block bits (M); -1 = default (7 int, 6 float) */
+    int32_t inner_bits;   /* unrolled inner block bits (M: 1, 4, 6 or 7); -1 = 7 */
     int32_t flags;        /* reserved, must be 0 */
 } qbx_opts_t;
 
@@ -86,9 +86,11 @@ int qbx_device_info(qbx_ctx_t *ctx, qbx_devinfo_t *info);
  *
  * qbx_solve_i32: exact int32 arithmetic.  Requires
  *   bound = sum_i |Q_ii| + sum_{i<j} |Q_ij + Q_ji| <= 2^31 - 2, else QBX_ERANGE.
- * qbx_solve_f64: FP32 block filter + FP64 exact re-walk; Q given as doubles (float32 and
- *   integer inputs convert exactly).  Integer-valued Q gives exact results while
- *   bound < 2^53.  1 <= n <= 56 for both. */
+ * qbx_solve_f64: Q given as doubles (float32 and integer inputs convert exactly); the int32
+ *   walk runs over tables quantized to 2^s units as a filter with a proven error bound
+ *   (n(n+1)/4 units per state), and the blocks it passes are walked exactly in FP64.  The
+ *   answer is the FP64 (E, rank) minimum.  Requires finite entries with bound <= 1e300,
+ *   else QBX_ERANGE.  1 <= n <= 56 for both. */
 int qbx_solve_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opts,
                   qbx_result_t *out);
 int qbx_solve_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts,

@@ -28,6 +28,7 @@ def emu():
                                     P(ctypes.c_int64), P(ctypes.c_uint64), P(ctypes.c_uint64)]
     L.qbx_emu_solve_f64.argtypes = [P(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    L.qbx_emu_take_stats.argtypes = [P(ctypes.c_uint64), P(ctypes.c_uint64)]
     L.qbx_emu_state_to_rank.restype = ctypes.c_uint64
     L.qbx_emu_state_to_rank.argtypes = [ctypes.c_uint64, ctypes.c_int]
     L.qbx_emu_rank_to_state.restype = ctypes.c_uint64
@@ -86,16 +87,54 @@ def test_medium_against_brute(emu, M):
                 assert run_i32(emu, Q, B, L, M) == ref, (n, B, L, M)
 
 
-@pytest.mark.parametrize("M", [1, 4, 6])
+def _float_case(rng, n, trial):
+    kind = trial % 6
+    if kind == 0:      # exact ties: integer-valued float matrix
+        return random_instance(rng, n, "maxcut").astype(np.float64)
+    if kind == 1:      # one huge entry: the small ones quantize to zero, the filter passes much
+        Q = rng.standard_normal((n, n)) * 1e-3
+        Q[0, n - 1] = 1e9
+        return Q
+    if kind == 2:      # tiny magnitudes (scale far above 2^31)
+        return rng.standard_normal((n, n)) * 1e-200
+    if kind == 3:      # near-ties: two copies of a state's energy within 1e-12
+        Q = np.round(rng.standard_normal((n, n)) * 8) / 8
+        Q[1, 1] += 1e-12
+        return Q
+    return rng.standard_normal((n, n)).astype(np.float32).astype(np.float64)
+
+
+def take_stats(emu):
+    f, r = ctypes.c_uint64(), ctypes.c_uint64()
+    emu.qbx_emu_take_stats(ctypes.byref(f), ctypes.byref(r))
+    return f.value, r.value
+
+
+@pytest.mark.parametrize("M", [1, 4, 6, 7])
 def test_float_filter_walk(emu, M):
+    """int32 filter over quantized tables + FP64 re-walk gives the FP64 brute-force answer,
+    through every path of the deferred-candidate buffer (overflow flushes included)."""
     rng = np.random.default_rng(31 + M)
+    take_stats(emu)
+    flushes = rewalks = 0
     for trial in range(12):
         n = int(rng.integers(max(M, 4), 17))
-        Q = rng.standard_normal((n, n)).astype(np.float32).astype(np.float64)
-        if trial % 3 == 0:   # exact ties: integer-valued float matrix
-            Q = random_instance(rng, n, "maxcut").astype(np.float64)
+        Q = _float_case(rng, n, trial)
         B = [min(n, 20), 0, n // 2][trial % 3]
         E, s = oracle.brute_f64(Q, B)
         for L in sorted({M, min(n, M + 4), min(n, M + 9)}):
             E2, s2 = run_f64(emu, Q, B, L, M)
-            assert s2 == s and E2 == pytest.approx(E, abs=1e-9), (n, B, L, M)
+            # FP64 sums in another order differ by rounding only: |dE| <= 1e-13 sum|Q|
+            assert s2 == s and E2 == pytest.approx(E, rel=0, abs=1e-13 * np.abs(Q).sum() + 1e-300), \
+                (n, B, L, M, trial)
+            f, r = take_stats(emu)
+            flushes, rewalks = flushes + f, rewalks + r
+    # a lane with hundreds of blocks that tie after quantization overflows its record buffer
+    n = 16
+    Q = _float_case(rng, n, 1)
+    E, s = oracle.brute_f64(Q, 16)
+    E2, s2 = run_f64(emu, Q, 16, min(n, M + 9), M)
+    assert s2 == s and E2 == pytest.approx(E, rel=0, abs=1e-13 * np.abs(Q).sum())
+    f, r = take_stats(emu)
+    assert f > 0 and r > 0
+    assert rewalks > 0
