@@ -84,3 +84,16 @@ def test_parallel_validation():
 def test_state_bits():
     assert engine.state_bits(0b1011, 4).tolist() == [1, 0, 1, 1]
     assert engine.state_bits(1, 3).dtype == np.int32
+
+
+def test_graft_entry_build_wiring(monkeypatch):
+    """build() calls the native build and the oracle recipe, then imports the package
+    (compilers replaced by stubs here; the real build runs outside the test suite)."""
+    import __graft_entry__ as g
+    import build_native
+    import oracle
+    calls = []
+    monkeypatch.setattr(build_native, "build_all", lambda: calls.append("native") or [])
+    monkeypatch.setattr(oracle, "build", lambda *a, **k: calls.append("oracle"))
+    g.build()
+    assert calls == ["native", "oracle"]
