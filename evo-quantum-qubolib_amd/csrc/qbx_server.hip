@@ -112,7 +112,7 @@ struct qbx_server {
     int stage_next = 0;
     // stats
     std::atomic<uint64_t> n_submitted{0}, n_completed{0}, n_steps{0}, n_chunks{0}, n_admissions{0};
-    double busy_ms = 0;
+    std::atomic<double> busy_ms{0.0};   // written by the worker, read by qbx_server_stats
     int worker_rc = QBX_OK;
     std::string worker_msg;
 
@@ -212,7 +212,8 @@ int complete_step(qbx_server* S, int64_t k) {
     const int s = (int)(k & 1);
     HIP_TRY(hipEventSynchronize(S->ev_step[s]));
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, S->ev_start[s], S->ev_step[s]) == hipSuccess) S->busy_ms += ms;
+    if (hipEventElapsedTime(&ms, S->ev_start[s], S->ev_step[s]) == hipSuccess)
+        S->busy_ms.store(S->busy_ms.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
     std::vector<Req*> keep;
     keep.reserve(S->active.size());
     for (Req* r : S->active) {
@@ -566,7 +567,7 @@ int qbx_server_stats(qbx_server_t* S, qbx_server_stats_t* out) {
     out->steps = S->n_steps;
     out->chunks = S->n_chunks;
     out->admissions = S->n_admissions;
-    out->busy_ms = S->busy_ms;
+    out->busy_ms = S->busy_ms.load(std::memory_order_relaxed);
     return QBX_OK;
 }
 

@@ -74,6 +74,49 @@ def test_float_exact_ties(gpu_ctx):
         assert (got.E, got.state) == (E, s)
 
 
+def _float_stress(rng, n, kind):
+    if kind == 0:      # integer-valued: exact ties everywhere, resolved by rank in FP64
+        return random_instance(rng, n, "maxcut").astype(np.float64)
+    if kind == 1:      # one huge entry: small terms quantize to zero, hundreds of blocks tie
+        Q = rng.standard_normal((n, n)) * 1e-3
+        Q[0, n - 1] = 1e9
+        return Q
+    if kind == 2:      # tiny magnitudes
+        return rng.standard_normal((n, n)) * 1e-200
+    if kind == 3:      # near-ties 1e-12 apart
+        Q = np.round(rng.standard_normal((n, n)) * 8) / 8
+        Q[1, 1] += 1e-12
+        return Q
+    return rng.standard_normal((n, n)).astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("inner", [1, 4, 6, 7])
+def test_float_filter_kernel_stress(gpu_ctx, inner):
+    """The float kernel (int32 filter on quantized tables, LDS candidate records, FP64
+    re-walks) on matrices built to stress the filter, at tilings with many blocks per lane,
+    called directly (no routing to the int path)."""
+    rng = np.random.default_rng(40 + inner)
+    for trial in range(10):
+        n = int(rng.integers(max(inner, 8), 21))
+        Q = _float_stress(rng, n, trial % 5)
+        B = [None, 0, n // 2][trial % 3]
+        E, st = oracle.brute_f64(Q, B)
+        tol = 1e-13 * np.abs(Q).sum() + 1e-300
+        for L in sorted({inner, min(n, inner + 9), min(n, inner + 12)}):
+            r = gpu_ctx.solve_f64(Q, tie_B=B, walk_bits=L, inner_bits=inner)
+            assert r.status == 0 and r.state == st and abs(r.E - E) <= tol, (trial, n, B, L, inner)
+
+
+def test_float_out_of_range_is_reported(gpu_ctx):
+    Q = np.zeros((5, 5))
+    Q[0, 0] = np.inf
+    with pytest.raises(OverflowError):
+        gpu_ctx.solve_f64(Q)
+    Q[0, 0] = np.nan
+    with pytest.raises(OverflowError):
+        gpu_ctx.solve_f64(Q)
+
+
 def test_large_integers_take_exact_fp64_path(gpu_ctx):
     rng = np.random.default_rng(8)
     for trial in range(6):
