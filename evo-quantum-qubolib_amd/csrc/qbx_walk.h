@@ -290,9 +290,15 @@ QBX_HD void block_state_i32(const int32_t* __restrict__ W, const int32_t* __rest
 // One block of an unrolled group: the transition into block c = c0 + J (J > 0; its outer
 // bit index ctz(J) is a compile-time constant), the block's minimum, and the best-key update.
 // Returns false when c is past the lane's last block.
+// The lane's best block so far, (E, K) ordered lexicographically.
+struct LaneBest {
+    int32_t E;
+    uint32_t K;
+};
+
 template <int M, int OMAX, int J>
 QBX_HD bool group_step_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* dl,
-                           uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, int64_t& best) {
+                           uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best) {
     const uint32_t c = c0 + (uint32_t)J;
     if (J > 0) {
         if (c >= nblk) return false;
@@ -303,15 +309,18 @@ QBX_HD bool group_step_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
         trans_static_i32<M, OMAX, i>(s, W + (M + i) * LDW, up);
     }
     const int32_t mn = block_min_i32<M>(s.E0, s.f, s.nf, dl);
-    const uint32_t K = block_key(c, SBk, pmaskS);
-    const int64_t key = (int64_t)(((uint64_t)(uint32_t)mn << 32) | (uint64_t)K);
-    best = key < best ? key : best;
+    // rare after the first blocks: one compare per block on the common path, the block key
+    // only when the block ties or beats the lane's best
+    if (__builtin_expect(mn <= best.E, 0)) {
+        const uint32_t K = block_key(c, SBk, pmaskS);
+        if (mn < best.E || K < best.K) { best.E = mn; best.K = K; }
+    }
     return true;
 }
 
 template <int M, int OMAX, int... Js>
 QBX_HD void group_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* dl,
-                      uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, int64_t& best,
+                      uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best,
                       std::integer_sequence<int, Js...>) {
     (void)(group_step_i32<M, OMAX, Js>(s, W, dl, c0, nblk, SBk, pmaskS, best) && ...);
 }
@@ -401,7 +410,7 @@ QBX_HD int64_t lane_best_block_i32(const int32_t* __restrict__ W, const int32_t*
 
     const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
     const uint32_t pmaskS = (parity64(p) && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
-    int64_t best = KEY_NONE;
+    LaneBest best{0x7fffffff, 0xffffffffu};
     const uint32_t nblk = 1u << O;
     for (uint32_t c0 = 0;;) {
         group_i32<M, OMAX>(s, W, dl, c0, nblk, SBk, pmaskS, best,
@@ -412,7 +421,7 @@ QBX_HD int64_t lane_best_block_i32(const int32_t* __restrict__ W, const int32_t*
         const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
         trans_dyn_i32<M, OMAX>(s, W + (M + i) * LDW, i, up);
     }
-    return best;
+    return (int64_t)(((uint64_t)(uint32_t)best.E << 32) | (uint64_t)best.K);
 }
 
 // Lowest inner rank of the minimisers inside block K of lane lam (whose minimum is E):
