@@ -14,7 +14,8 @@ import numpy as np
 
 __all__ = [
     "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
-    "ServerStats", "BatchPlan", "KINDS", "MODES", "load_library", "get_context",
+    "ServerStats", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
+    "tuning_path",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libqbx.so"
@@ -180,6 +181,37 @@ def load_library(path: Path | str | None = None) -> ctypes.CDLL:
         return lib
 
 
+def tuning_path(arch: str, path=None) -> Path | None:
+    """The tiling table to load: `path`, else $QUBO_LIB_AMD_TUNING ("off": none), else
+    tuning/<arch>.json next to this module; None when the file does not exist."""
+    env = os.environ.get("QUBO_LIB_AMD_TUNING")
+    if path is None and env:
+        if env.lower() == "off":
+            return None
+        path = env
+    if path is None:
+        path = TUNING_DIR / f"{arch}.json"
+    path = Path(path)
+    return path if path.exists() else None
+
+
+def read_tuning(path) -> dict:
+    """{(kind, mode): walk bits per n} from a table written by qubo_lib_amd.tune.  Raises
+    ValueError on an unknown kind/mode or an entry outside -1 .. n."""
+    import json
+    data = json.loads(Path(path).read_text())
+    out = {}
+    for key, table in data.get("walk_bits", {}).items():
+        kind, _, mode = key.partition("/")
+        if kind not in KINDS or mode not in MODES:
+            raise ValueError(f"{path}: unknown table {key!r}")
+        table = [int(v) for v in table]
+        if len(table) > 57 or any(v < -1 or v > n for n, v in enumerate(table)):
+            raise ValueError(f"{path}: table {key!r} has an entry outside -1..n")
+        out[(kind, mode)] = table
+    return out
+
+
 def _check(lib, rc: int):
     if rc == QBX_OK:
         return
@@ -214,22 +246,11 @@ class Context:
         """Install the per-n walk-bit tables written by qubo_lib_amd.tune for this device's
         architecture.  Looked up in order: `path`, $QUBO_LIB_AMD_TUNING ("off" disables),
         qubo_lib_amd/tuning/<arch>.json.  A missing file keeps the built-in rule."""
-        env = os.environ.get("QUBO_LIB_AMD_TUNING")
-        if path is None and env:
-            if env.lower() == "off":
-                return None
-            path = env
+        path = tuning_path(self.device_info()["name"].split(":")[0], path)
         if path is None:
-            arch = self.device_info()["name"].split(":")[0]
-            path = TUNING_DIR / f"{arch}.json"
-        path = Path(path)
-        if not path.exists():
             return None
-        import json
-        data = json.loads(path.read_text())
-        for key, table in data.get("walk_bits", {}).items():
-            kind, mode = key.split("/")
-            self.set_walk_table(kind, mode, [int(v) for v in table])
+        for (kind, mode), table in read_tuning(path).items():
+            self.set_walk_table(kind, mode, table)
         self.tuning_file = path
         return path
 

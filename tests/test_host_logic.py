@@ -97,3 +97,32 @@ def test_graft_entry_build_wiring(monkeypatch):
     monkeypatch.setattr(oracle, "build", lambda *a, **k: calls.append("oracle"))
     g.build()
     assert calls == ["native", "oracle"]
+
+
+def test_tuning_table_files(tmp_path, monkeypatch):
+    """The committed gfx950 table parses and is valid for every n; lookup honours the
+    environment override and 'off'; malformed tables are rejected."""
+    import json
+
+    from qubo_lib_amd import _native
+    monkeypatch.delenv("QUBO_LIB_AMD_TUNING", raising=False)
+    path = _native.tuning_path("gfx950")
+    assert path is not None and path.name == "gfx950.json"
+    tables = _native.read_tuning(path)
+    assert set(tables) == {(k, m) for k in ("i32", "f64") for m in ("single", "batch")}
+    for (kind, mode), t in tables.items():
+        assert len(t) == 57
+        tuned = [n for n, v in enumerate(t) if v >= 0]
+        assert tuned and all(1 <= t[n] <= n for n in tuned)
+    assert _native.tuning_path("gfx000") is None
+    monkeypatch.setenv("QUBO_LIB_AMD_TUNING", "off")
+    assert _native.tuning_path("gfx950") is None
+    bad = tmp_path / "bad.json"
+    bad.write_text(json.dumps({"walk_bits": {"i32/single": [-1, 5]}}))
+    monkeypatch.setenv("QUBO_LIB_AMD_TUNING", str(bad))
+    assert _native.tuning_path("gfx950") == bad
+    with pytest.raises(ValueError):
+        _native.read_tuning(bad)
+    bad.write_text(json.dumps({"walk_bits": {"i16/single": [-1]}}))
+    with pytest.raises(ValueError):
+        _native.read_tuning(bad)
