@@ -249,6 +249,7 @@ def batch(args, ctx, devinfo):
     same = all((a.E_int, a.state) == (b.E_int, b.state) == (e.E_int, e.state)
                for a, b, e in zip(res, plain, res_e))
     states = float(sum(2.0 ** q.shape[0] for q in Qs))
+    drawn = device_drawn_random_batch(args, ctx)
     return {
         "metric": "batched solves/s (4096-instance uneven mix)",
         "value": len(Qs) / wall_b,
@@ -266,8 +267,34 @@ def batch(args, ctx, devinfo):
                   "p99_ms": float(np.percentile(lat, 99) * 1e3)},
         "speedup_vs_plain": wall_p / wall_b,
         "identical_answers": same,
+        "random_qubo_drawn_on_device": drawn,
         "states_per_s": states / wall_b,
     }
+
+
+def device_drawn_random_batch(args, ctx, count: int = 4096) -> dict:
+    """RandomQUBO(seed, N ~ U{16..32}, int16 in [-100, 100)) x count, end to end: matrices
+    drawn by numpy on the host and staged vs drawn on the device (same bits)."""
+    from qubo_lib_amd.problems import RandomQUBO
+    rng = np.random.default_rng(args.seed)
+    seeds = [int(s) for s in rng.integers(0, 2 ** 31, count)]
+    ns = [int(n) for n in rng.integers(16, 33, count)]
+    probs = [RandomQUBO.init_random_instance(seed=s, N=n, dtype=np.int16) for s, n in zip(seeds, ns)]
+    ctx.random_qubo_batch(seeds[:64], ns[:64], -100, 100, np.int16)        # warm-up
+    t0 = time.perf_counter()
+    mats = [p.build_Q_matrix() for p in probs]
+    t1 = time.perf_counter()
+    host = ctx.batch([m.astype(np.int32) for m in mats], kind="i32")
+    t2 = time.perf_counter()
+    dev = ctx.random_qubo_batch(seeds, ns, -100, 100, np.int16)
+    t3 = time.perf_counter()
+    same = all((a.E_int, a.state) == (b.E_int, b.state) for a, b in zip(host, dev))
+    return {"workload": f"{count} RandomQUBO int16 [-100,100), N ~ U{{16..32}}",
+            "host_drawn": {"solves_per_s": count / (t2 - t0), "numpy_ms": (t1 - t0) * 1e3,
+                           "solve_ms": (t2 - t1) * 1e3},
+            "device_drawn": {"solves_per_s": count / (t3 - t2), "ms": (t3 - t2) * 1e3,
+                             "kernel_ms": dev[0].kernel_ms},
+            "identical_answers": same}
 
 
 def serve(args, ctx, devinfo):

@@ -49,7 +49,9 @@ def build_lib(out: Path = LIB_OUT) -> Path:
 
 def build_emu(out: Path = EMU_OUT) -> Path:
     out.parent.mkdir(parents=True, exist_ok=True)
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{CSRC}", "-o", str(out),
+    # no floating-point contraction: the generator's float scaling is one rounded multiply
+    # and one rounded add, as numpy evaluates it
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", f"-I{CSRC}", "-o", str(out),
           str(CSRC / "qbx_emu.cpp")])
     return out
 

@@ -56,6 +56,11 @@ struct Layout {
     std::vector<Group> groups;   // one persistent launch per variant
     size_t off_inst = 0, off_ranges = 0, stage_bytes = 0, off_red = 0, off_res = 0, off_ticket = 0;
     size_t total_bytes = 0;
+    // matrices generated on the device (RandomQUBO): PCG64 states staged instead of Q
+    bool gen = false;
+    int gen_dtype = 0;
+    int64_t gen_low = 0, gen_high = 0;
+    size_t off_rng = 0;
 };
 }  // namespace qbx
 
@@ -97,7 +102,7 @@ int ensure_capacity(qbx_ctx* c, size_t dbytes, size_t hbytes) {
 }
 
 int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_opts_t* opts, bool single,
-                Layout& lay) {
+                Layout& lay, bool gen = false) {
     if (count <= 0) return set_err(QBX_EINVAL, "count must be positive, got %d", count);
     if (!ns) return set_err(QBX_EINVAL, "null argument");
     qbx_opts_t o = {-1, -1, -1, 0};
@@ -113,6 +118,10 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     off = align256(off + sizeof(InstDev) * count);
     lay.off_ranges = off;
     off = align256(off + sizeof(TileRange) * count);
+    lay.gen = gen;
+    lay.off_rng = off;
+    if (gen) off = align256(off + 4 * sizeof(uint64_t) * count);
+    const size_t q_begin = off;
     const PlanMode mode = single ? PLAN_SINGLE : PLAN_BATCH;
     for (int i = 0; i < count; ++i) {
         Plan& p = plans[i];
@@ -121,7 +130,9 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
         p.off_Q = off;
         off = align256(off + lay.esize * (size_t)p.n * p.n);
     }
-    lay.stage_bytes = off;
+    // staged from the host: [instances | ranges | Q data], or [instances | ranges | PCG64
+    // states] when the matrices are generated on the device
+    lay.stage_bytes = gen ? q_begin : off;
     for (int i = 0; i < count; ++i) {
         Plan& p = plans[i];
         p.off_W = off;
@@ -219,6 +230,12 @@ int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const c
             hipEvent_t ev_begin, hipEvent_t ev_end) {
     HIP_TRY(hipMemcpyAsync(dv, h_stage, lay.stage_bytes, hipMemcpyHostToDevice, stream));
     if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, stream));
+    if (lay.gen) {
+        hipLaunchKernelGGL(qbx_gen_random, dim3((unsigned)((lay.count + 63) / 64)), dim3(64), 0, stream,
+                           (const InstDev*)(dv + lay.off_inst), lay.count, (const uint64_t*)(dv + lay.off_rng),
+                           lay.gen_low, lay.gen_high, lay.gen_dtype);
+        HIP_TRY(hipGetLastError());
+    }
     unsigned long long* dticket = (unsigned long long*)(dv + lay.off_ticket);
     hipLaunchKernelGGL(qbx_prep, dim3(lay.count), dim3(256), 0, stream, (InstDev*)(dv + lay.off_inst),
                        (const uint32_t*)nullptr, dticket, (const InstDev*)nullptr);
@@ -292,6 +309,34 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     return collect(lay, (const ResultDev*)h, ms, out);
+}
+
+// Device-generated RandomQUBO batches: the layout, the staged PCG64 states and (for
+// qbx_random_qubo_read) the matrices copied back.
+int gen_prepare(qbx_ctx* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low, int64_t high,
+                int dtype, const qbx_opts_t* opts, Layout& lay) {
+    if (!c) return set_err(QBX_EINVAL, "null context");
+    if (!rng || !ns) return set_err(QBX_EINVAL, "null argument");
+    if (!gen_valid(low, high, dtype))
+        return set_err(QBX_EINVAL, "unsupported generator bounds [%lld, %lld) for dtype %d", (long long)low,
+                       (long long)high, dtype);
+    const int kind = (dtype == GEN_INT16 || dtype == GEN_INT32) ? KIND_I32 : KIND_F64;
+    int rc = make_layout(c, kind, count, ns, opts, false, lay, true);
+    if (rc) return rc;
+    lay.gen_dtype = dtype;
+    lay.gen_low = low;
+    lay.gen_high = high;
+    rc = ensure_capacity(c, lay.total_bytes, std::max(lay.total_bytes, sizeof(ResultDev) * (size_t)count));
+    if (rc) return rc;
+    stage_descriptors(lay, c->hbuf, c->dbuf);
+    uint64_t* hr = (uint64_t*)(c->hbuf + lay.off_rng);
+    for (int i = 0; i < count; ++i) {
+        hr[4 * i + 0] = rng[i].state_hi;
+        hr[4 * i + 1] = rng[i].state_lo;
+        hr[4 * i + 2] = rng[i].inc_hi;
+        hr[4 * i + 3] = rng[i].inc_lo;
+    }
+    return QBX_OK;
 }
 
 void plan_free(qbx_batch_plan* P) {
@@ -480,6 +525,48 @@ int qbx_batch_plan_destroy(qbx_batch_plan_t* P) {
         hipStreamSynchronize(P->ctx->stream);
     }
     plan_free(P);
+    return QBX_OK;
+}
+
+int qbx_batch_random_qubo(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low,
+                          int64_t high, int dtype, const qbx_opts_t* opts, qbx_result_t* out) {
+    if (!out) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    int rc = gen_prepare(c, count, rng, ns, low, high, dtype, opts, lay);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    rc = enqueue(c, c->stream, lay, c->dbuf, c->hbuf, c->hbuf, c->ev0, c->ev1);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return collect(lay, (const ResultDev*)c->hbuf, ms, out);
+}
+
+int qbx_random_qubo_read(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low,
+                         int64_t high, int dtype, void* Qout) {
+    if (!Qout) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    int rc = gen_prepare(c, count, rng, ns, low, high, dtype, nullptr, lay);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpyAsync(c->dbuf, c->hbuf, lay.stage_bytes, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(qbx_gen_random, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, c->stream,
+                       (const InstDev*)(c->dbuf + lay.off_inst), count, (const uint64_t*)(c->dbuf + lay.off_rng),
+                       low, high, dtype);
+    HIP_TRY(hipGetLastError());
+    // the Q section is contiguous from the first problem's matrix to the last one's
+    const size_t q0 = lay.plans[0].off_Q;
+    const Plan& pl = lay.plans[count - 1];
+    const size_t q1 = pl.off_Q + lay.esize * (size_t)pl.n * pl.n;
+    HIP_TRY(hipMemcpyAsync(c->hbuf + q0, c->dbuf + q0, q1 - q0, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    char* dst = (char*)Qout;
+    for (int i = 0; i < count; ++i) {
+        const size_t bytes = lay.esize * (size_t)lay.plans[i].n * lay.plans[i].n;
+        memcpy(dst, c->hbuf + lay.plans[i].off_Q, bytes);
+        dst += bytes;
+    }
     return QBX_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "qbx_walk.h"
+#include "qbx_gen.h"
 #include "qbx.h"
 
 namespace qbx {
@@ -186,6 +187,27 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         I.red->cand_count = 0u;
         I.red->done = 0u;
     }
+}
+
+// ------------------------------------------------------------ RandomQUBO generation
+// One thread per problem: the problem's matrix, drawn from its PCG64 state (4 words per
+// problem: state hi, lo, increment hi, lo) exactly as numpy draws it, written where the
+// host would have staged it (I.Q, in the element type of the problem's kind).
+static __global__ __launch_bounds__(64) void qbx_gen_random(const InstDev* insts, int count, const uint64_t* rng,
+                                                           int64_t low, int64_t high, int dtype) {
+    const int i = (int)(blockIdx.x * 64u + threadIdx.x);
+    if (i >= count) return;
+    const InstDev& I = insts[i];
+    Pcg64 g;
+    g.hi = rng[4 * i + 0];
+    g.lo = rng[4 * i + 1];
+    g.inc_hi = rng[4 * i + 2];
+    g.inc_lo = rng[4 * i + 3];
+    g.buf32 = 0;
+    g.buf16 = 0;
+    g.has32 = g.has16 = 0;
+    if (I.kind == KIND_I32) gen_random_qubo<int32_t>(g, I.n, low, high, dtype, (int32_t*)I.Q);
+    else gen_random_qubo<double>(g, I.n, low, high, dtype, (double*)I.Q);
 }
 
 // ------------------------------------------------------------- tile commit / finalize

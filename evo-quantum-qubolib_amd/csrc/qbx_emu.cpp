@@ -11,6 +11,7 @@
 #include <vector>
 #define QBX_EMU_STATS 1
 #include "qbx_walk.h"
+#include "qbx_gen.h"
 
 using namespace qbx;
 
@@ -132,6 +133,18 @@ int qbx_emu_solve_f64(const double* Q, int n, int B, int L, int M, double* E, ui
     }
     if (rc == 0) *state = rank_to_state(*rank, B);
     return rc;
+}
+
+// RandomQUBO matrix from a PCG64 state (state hi, lo, increment hi, lo), as the device
+// generator draws it: int32 elements for integer dtypes, double for float dtypes.
+int qbx_emu_random_qubo(const uint64_t* st, int n, int64_t low, int64_t high, int dtype, void* out) {
+    if (!gen_valid(low, high, dtype)) return -1;
+    Pcg64 g;
+    g.hi = st[0]; g.lo = st[1]; g.inc_hi = st[2]; g.inc_lo = st[3];
+    g.buf32 = 0; g.buf16 = 0; g.has32 = g.has16 = 0;
+    if (dtype == GEN_INT16 || dtype == GEN_INT32) gen_random_qubo<int32_t>(g, n, low, high, dtype, (int32_t*)out);
+    else gen_random_qubo<double>(g, n, low, high, dtype, (double*)out);
+    return 0;
 }
 
 // float filter path counters since the last call (buffer overflows, FP64 block re-walks)

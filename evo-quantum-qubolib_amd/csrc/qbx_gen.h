@@ -1,0 +1,163 @@
+// qbx_gen.h -- RandomQUBO matrices generated where they are solved, bit for bit as the
+// reference generates them on the host (src/qubo_lib/problems/random_qubo.py:19-33:
+// Generator(PCG64(seed)).integers(low, high, (N, N), dtype) for integer dtypes,
+// low + random((N, N), dtype) * (high - low) for float dtypes).  Shared by the device
+// kernel and the host build used by the CPU tests.
+//
+// The bit generator is PCG64 (128-bit LCG, XSL-RR output): each draw first advances
+// s <- s * M + inc (mod 2^128), then outputs rotr64(hi(s) ^ lo(s), s >> 122).  The host
+// seeds it (SeedSequence) and passes the 128-bit state and increment.  Draws narrower than
+// 64 bits come from buffers, low half first: 32-bit values split one 64-bit output; 16-bit
+// values (int16 bounds) split one 32-bit value.  Bounded integers use Lemire's
+// multiply-and-reject method on the draw width (16 bits for int16, 32 bits for int32 and
+// int64 with a range below 2^32); a range of one value draws nothing.  Uniform floats are
+// (u32 >> 8) 2^-24 (float32) and (u64 >> 11) 2^-53 (float64), scaled with one rounded
+// multiply and one rounded add (not a fused multiply-add), as numpy evaluates it.  All of this was
+// checked against numpy 2.2's Generator on the CPU (tests/test_generators_device.py).
+#pragma once
+#include <stdint.h>
+
+#include "qbx_walk.h"
+
+namespace qbx {
+
+enum { GEN_INT16 = 16, GEN_INT32 = 32, GEN_FLOAT32 = -32, GEN_FLOAT64 = -64 };
+
+struct Pcg64 {
+    uint64_t hi, lo, inc_hi, inc_lo;
+    uint64_t buf32;   // pending upper half of a 64-bit output
+    uint32_t buf16;   // pending upper half of a 32-bit value
+    int has32, has16;
+};
+
+QBX_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+QBX_HD uint64_t pcg64_next(Pcg64& g) {
+    const uint64_t Mhi = 0x2360ED051FC65DA4ull, Mlo = 0x4385DF649FCCF645ull;
+    const uint64_t lo = g.lo * Mlo;
+    const uint64_t hi = mulhi64(g.lo, Mlo) + g.lo * Mhi + g.hi * Mlo;
+    const uint64_t nlo = lo + g.inc_lo;
+    const uint64_t nhi = hi + g.inc_hi + (nlo < lo ? 1ull : 0ull);
+    g.lo = nlo;
+    g.hi = nhi;
+    const uint64_t x = nhi ^ nlo;
+    const unsigned r = (unsigned)(nhi >> 58);
+    return (x >> r) | (x << ((64u - r) & 63u));
+}
+
+QBX_HD uint32_t pcg64_u32(Pcg64& g) {
+    if (g.has32) {
+        g.has32 = 0;
+        return (uint32_t)g.buf32;
+    }
+    const uint64_t v = pcg64_next(g);
+    g.buf32 = v >> 32;
+    g.has32 = 1;
+    return (uint32_t)v;
+}
+
+QBX_HD uint16_t pcg64_u16(Pcg64& g) {
+    if (g.has16) {
+        g.has16 = 0;
+        return (uint16_t)g.buf16;
+    }
+    const uint32_t v = pcg64_u32(g);
+    g.buf16 = v >> 16;
+    g.has16 = 1;
+    return (uint16_t)v;
+}
+
+// value in [0, rng] (rng = high - low - 1 >= 1)
+QBX_HD uint32_t lemire16(Pcg64& g, uint32_t rng) {
+    const uint32_t excl = rng + 1u;
+    uint32_t m = (uint32_t)pcg64_u16(g) * excl;
+    uint32_t left = m & 0xffffu;
+    if (left < excl) {
+        const uint32_t thr = (0x10000u - excl) % excl;
+        while (left < thr) {
+            m = (uint32_t)pcg64_u16(g) * excl;
+            left = m & 0xffffu;
+        }
+    }
+    return m >> 16;
+}
+
+QBX_HD uint64_t lemire32(Pcg64& g, uint64_t rng) {
+    const uint64_t excl = rng + 1ull;
+    uint64_t m = (uint64_t)pcg64_u32(g) * excl;
+    uint64_t left = m & 0xffffffffull;
+    if (left < excl) {
+        const uint64_t thr = (0x100000000ull - excl) % excl;
+        while (left < thr) {
+            m = (uint64_t)pcg64_u32(g) * excl;
+            left = m & 0xffffffffull;
+        }
+    }
+    return m >> 32;
+}
+
+// low + r * span with two roundings, never fused into one multiply-add (which rounds once
+// and differs from numpy in the last bit): contraction is switched off for these
+// functions on the device compiler, and the host build uses -ffp-contract=off.
+QBX_HD float gen_scale_f32(float lo, float r, float span) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const float t = r * span;
+    return lo + t;
+}
+QBX_HD double gen_scale_f64(double lo, double r, double span) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const double t = r * span;
+    return lo + t;
+}
+
+// True when (low, high, dtype) is a supported generator configuration.
+QBX_HD bool gen_valid(int64_t low, int64_t high, int dtype) {
+    if (dtype == GEN_INT16) return high > low && low >= -32768 && high <= 32768;
+    if (dtype == GEN_INT32) return high > low && low >= -2147483648ll && high <= 2147483648ll;
+    if (dtype == GEN_FLOAT32 || dtype == GEN_FLOAT64)
+        return low >= -(1ll << 53) && high <= (1ll << 53) && low <= high;
+    return false;
+}
+
+// One matrix, row-major, in the element type the solver takes (int32 for integer dtypes,
+// double for float dtypes: float32 values convert exactly).
+template <class T>
+QBX_HD void gen_random_qubo(Pcg64 g, int n, int64_t low, int64_t high, int dtype, T* out) {
+    g.has32 = 0;
+    g.has16 = 0;
+    const uint64_t cells = (uint64_t)n * (uint64_t)n;
+    if (dtype == GEN_INT16 || dtype == GEN_INT32) {
+        const uint64_t rng = (uint64_t)(high - low) - 1ull;
+        for (uint64_t i = 0; i < cells; ++i) {
+            uint64_t v = 0;
+            if (rng == 0) v = 0;
+            else if (dtype == GEN_INT16) v = lemire16(g, (uint32_t)rng);
+            else v = lemire32(g, rng);
+            out[i] = (T)(low + (int64_t)v);
+        }
+    } else if (dtype == GEN_FLOAT32) {
+        const float lo = (float)low, span = (float)(high - low);
+        for (uint64_t i = 0; i < cells; ++i) {
+            const float r = (float)(pcg64_u32(g) >> 8) * 5.9604644775390625e-08f;   // 2^-24
+            out[i] = (T)gen_scale_f32(lo, r, span);
+        }
+    } else {
+        const double lo = (double)low, span = (double)(high - low);
+        for (uint64_t i = 0; i < cells; ++i) {
+            const double r = (double)(pcg64_next(g) >> 11) * 1.1102230246251565e-16;   // 2^-53
+            out[i] = (T)gen_scale_f64(lo, r, span);
+        }
+    }
+}
+
+}  // namespace qbx

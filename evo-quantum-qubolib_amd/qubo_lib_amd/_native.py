@@ -14,7 +14,7 @@ import numpy as np
 
 __all__ = [
     "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
-    "ServerStats", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
+    "ServerStats", "Pcg64State", "GEN_DTYPES", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
     "tuning_path",
 ]
 
@@ -82,6 +82,18 @@ class DevInfo(ctypes.Structure):
     ]
 
 
+class Pcg64State(ctypes.Structure):
+    _fields_ = [
+        ("state_hi", ctypes.c_uint64),
+        ("state_lo", ctypes.c_uint64),
+        ("inc_hi", ctypes.c_uint64),
+        ("inc_lo", ctypes.c_uint64),
+    ]
+
+
+GEN_DTYPES = {"int16": 16, "int32": 32, "float32": -32, "float64": -64}
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [
         ("max_n", ctypes.c_int32),
@@ -135,6 +147,10 @@ _SIGNATURES = {
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
     "qbx_batch_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
+    "qbx_batch_random_qubo": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
+                                             ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_random_qubo_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
+                                            ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]),
     "qbx_set_walk_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int32),
                                           ctypes.c_int]),
     "qbx_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(Opts),
@@ -343,6 +359,52 @@ class Context:
         if rc not in (QBX_OK, QBX_ERANGE):
             _check(self.lib, rc)
         return list(out)
+
+    # -- RandomQUBO batches drawn on the device ----------------------------------------
+    @staticmethod
+    def _gen_args(seeds, ns, dtype):
+        from numpy.random import PCG64
+        states = (Pcg64State * len(seeds))()
+        for st, seed in zip(states, seeds):
+            s = PCG64(seed).state["state"]
+            st.state_hi, st.state_lo = s["state"] >> 64, s["state"] & ((1 << 64) - 1)
+            st.inc_hi, st.inc_lo = s["inc"] >> 64, s["inc"] & ((1 << 64) - 1)
+        nsa = np.ascontiguousarray(np.asarray(ns, dtype=np.int32))
+        if nsa.size != len(seeds):
+            raise ValueError("one N per seed")
+        key = np.dtype(dtype).name
+        if key not in GEN_DTYPES:
+            raise TypeError(f"RandomQUBO dtype {key} cannot be drawn on the device")
+        return states, nsa, GEN_DTYPES[key]
+
+    def random_qubo_batch(self, seeds, ns, low=-100, high=100, dtype=np.float32, **kw):
+        """Solve RandomQUBO(seed, N, low, high, dtype) instances whose matrices are drawn on
+        the device (bit for bit numpy's); returns a list of Result."""
+        states, nsa, code = self._gen_args(seeds, ns, dtype)
+        out = (Result * len(seeds))()
+        o = _opts(**kw)
+        with self.lock:
+            rc = self.lib.qbx_batch_random_qubo(self._h, len(seeds), states, nsa.ctypes.data_as(_P(ctypes.c_int32)),
+                                                int(low), int(high), code, ctypes.byref(o), out)
+        if rc not in (QBX_OK, QBX_ERANGE):
+            _check(self.lib, rc)
+        return list(out)
+
+    def random_qubo_read(self, seeds, ns, low=-100, high=100, dtype=np.float32) -> list:
+        """The matrices random_qubo_batch would solve, drawn on the device and copied back
+        (int32 for integer dtypes, float64 for float dtypes)."""
+        states, nsa, code = self._gen_args(seeds, ns, dtype)
+        et = np.int32 if code > 0 else np.float64
+        buf = np.empty(int((nsa.astype(np.int64) ** 2).sum()), dtype=et)
+        with self.lock:
+            rc = self.lib.qbx_random_qubo_read(self._h, len(seeds), states, nsa.ctypes.data_as(_P(ctypes.c_int32)),
+                                               int(low), int(high), code, buf.ctypes.data_as(ctypes.c_void_p))
+        _check(self.lib, rc)
+        out, o = [], 0
+        for n in nsa:
+            out.append(buf[o:o + n * n].reshape(n, n))
+            o += n * n
+        return out
 
     def batch_plan(self, ns, kind: str = "i32", **kw) -> "BatchPlan":
         """Capture the launches of one batch shape in a HIP graph (see BatchPlan)."""

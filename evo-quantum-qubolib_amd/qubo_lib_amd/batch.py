@@ -19,15 +19,58 @@ from .solution import Solution
 from .state_vectors import StateVectors
 
 
-def solve_batch(problems, device: str = "cuda", B=None, walk_bits=None, inner_bits=None) -> list:
-    """Solve every problem exactly; returns one Solution per problem, in order."""
+def _drawable_on_device(problems) -> bool:
+    """Seeded RandomQUBO instances sharing bounds and a dtype the device generator draws."""
+    from ._native import GEN_DTYPES
+    from .problems.random_qubo import RandomQUBO
+    if not problems:
+        return False
+    first = problems[0]
+    for p in problems:
+        if not isinstance(p, RandomQUBO) or getattr(p, "seed", None) is None:
+            return False
+        if np.dtype(p.dtype).name not in GEN_DTYPES:
+            return False
+        if (p.low, p.high, np.dtype(p.dtype)) != (first.low, first.high, np.dtype(first.dtype)):
+            return False
+    if np.issubdtype(np.dtype(first.dtype), np.integer):
+        # integer draws are solved on the int32 path: its energy bound must hold for any draw
+        n = max(int(p.N) for p in problems)
+        m = max(abs(int(first.low)), abs(int(first.high) - 1))
+        if m * (n + n * (n - 1)) > engine.INT32_BOUND:
+            return False
+    return True
+
+
+def solve_batch(problems, device: str = "cuda", B=None, walk_bits=None, inner_bits=None,
+                draw_on_device=None) -> list:
+    """Solve every problem exactly; returns one Solution per problem, in order.
+
+    draw_on_device: seeded RandomQUBO batches can have their matrices drawn on the GPU
+    (bit for bit numpy's, random_qubo.py:19-33) instead of by numpy on the host; None
+    selects it for such batches of 64 or more problems."""
     dev = 0
     if isinstance(device, str) and device[:4] == "cuda":
         dev = int(device[5:]) if len(device) > 5 else 0
     else:
         raise ValueError(f"solve_batch runs on the GPU only: {device}")
-    mats = [np.asarray(p.build_Q_matrix(use_gpu=False)) for p in problems]
-    raws = engine.solve_matrices(mats, device=dev, tie_B=B, walk_bits=walk_bits, inner_bits=inner_bits)
+    drawable = _drawable_on_device(problems)
+    if draw_on_device and not drawable:
+        raise ValueError("draw_on_device needs seeded RandomQUBO problems with one (low, high, dtype)")
+    if draw_on_device is None:
+        draw_on_device = drawable and len(problems) >= 64
+    if draw_on_device:
+        from . import _native
+        ctx = _native.get_context(dev)
+        p0 = problems[0]
+        seeds, ns = [int(p.seed) for p in problems], [int(p.N) for p in problems]
+        res = ctx.random_qubo_batch(seeds, ns, p0.low, p0.high, p0.dtype, tie_B=B, walk_bits=walk_bits,
+                                    inner_bits=inner_bits)
+        mats = ctx.random_qubo_read(seeds, ns, p0.low, p0.high, p0.dtype)   # for the Ising offsets
+        raws = [engine.raw_from_native(r, integer=np.issubdtype(np.dtype(p0.dtype), np.integer)) for r in res]
+    else:
+        mats = [np.asarray(p.build_Q_matrix(use_gpu=False)) for p in problems]
+        raws = engine.solve_matrices(mats, device=dev, tie_B=B, walk_bits=walk_bits, inner_bits=inner_bits)
     params = json.dumps(dict(device=device, B=B, walk_bits=walk_bits, inner_bits=inner_bits))
     out = []
     for p, Q, r in zip(problems, mats, raws):

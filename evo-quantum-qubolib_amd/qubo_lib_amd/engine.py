@@ -113,6 +113,18 @@ def solve_matrix(Q, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None
     return _finish(p, r)
 
 
+def raw_from_native(r: _native.Result, integer: bool) -> RawResult:
+    """RawResult of a problem whose matrix never reached the host (device-drawn batches):
+    integer problems on the int32 path, float problems on the FP64 path."""
+    if r.status != 0:
+        raise OverflowError(f"N={r.n}: energy bound outside the {'int32' if integer else 'float'} path")
+    kind = "i32" if integer else "f64"
+    E = int(r.E_int) if integer else float(r.E)
+    info = dict(kernel_ms=r.kernel_ms, states=int(r.states), tiles=int(r.tiles), walk_bits=r.walk_bits,
+                inner_bits=r.inner_bits, lane_bits=r.lane_bits, path=kind, bound=None)
+    return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=int(r.n), tie_B=r.tie_B, kind=kind, info=info)
+
+
 def solve_matrices(Qs, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None):
     """Many problems, one persistent launch per arithmetic path.  Returns RawResults in
     input order.  `tie_B` may be a single value (None = min(N, 20) per problem)."""

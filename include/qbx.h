@@ -121,6 +121,28 @@ int qbx_batch_plan_run_f64(qbx_batch_plan_t *plan, const double *Qpacked, const 
                            qbx_result_t *out);
 int qbx_batch_plan_destroy(qbx_batch_plan_t *plan);
 
+/* ------------------------------------------------------- device-built RandomQUBO
+ * Batches of RandomQUBO instances whose matrices are drawn on the device, bit for bit as
+ * the reference draws them on the host: Generator(PCG64(seed)).integers(low, high, (N, N),
+ * dtype) for integer dtypes (high exclusive), low + random((N, N), dtype) * (high - low) for
+ * float dtypes (random_qubo.py:19-33; the reference's use_gpu=True path draws on the host
+ * and copies).  The caller seeds PCG64 (numpy's SeedSequence) and passes each instance's
+ * 128-bit state and increment.  Integer dtypes solve on the int32 path (QBX_ERANGE per
+ * problem when the energy bound does not fit), float dtypes on the FP64 path.
+ * qbx_random_qubo_read returns the generated matrices (packed n*n each, int32 for integer
+ * dtypes, double for float dtypes) instead of solving them. */
+#define QBX_GEN_INT16 16
+#define QBX_GEN_INT32 32
+#define QBX_GEN_FLOAT32 (-32)
+#define QBX_GEN_FLOAT64 (-64)
+typedef struct {
+    uint64_t state_hi, state_lo, inc_hi, inc_lo;
+} qbx_pcg64_t;
+int qbx_batch_random_qubo(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *ns,
+                          int64_t low, int64_t high, int dtype, const qbx_opts_t *opts, qbx_result_t *out);
+int qbx_random_qubo_read(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *ns,
+                         int64_t low, int64_t high, int dtype, void *Qout);
+
 /* ------------------------------------------------------------------- tiling
  * Walk bits (L: states per lane = 2^L, lanes per problem = 2^(n-L)) used when
  * opts->walk_bits = -1.  qbx_set_walk_table installs a per-n table measured by the

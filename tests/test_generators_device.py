@@ -1,0 +1,82 @@
+"""RandomQUBO matrices drawn where they are solved (csrc/qbx_gen.h) equal numpy's, bit for
+bit: the host build of the generator on the CPU, the device generator on the GPU."""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+from numpy.random import PCG64
+
+from qubo_lib_amd.problems import RandomQUBO
+
+EMU = Path(__file__).resolve().parent / "_build" / "libqbx_emu.so"
+CASES = [  # (seed, N, low, high, dtype)
+    (42, 44, -100, 100, np.int16), (7, 17, -127, 128, np.int16), (3, 5, -32768, 32768, np.int16),
+    (11, 30, -100, 100, np.int32), (12, 9, -2 ** 31, 2 ** 31, np.int32), (13, 6, 4, 5, np.int16),
+    (42, 37, -100, 100, np.float32), (5, 20, -3, 1000, np.float32), (6, 12, -100, 100, np.float64),
+    (8, 33, 0, 1, np.float32),
+]
+CODES = {np.int16: 16, np.int32: 32, np.float32: -32, np.float64: -64}
+
+
+def _state(seed):
+    s = PCG64(seed).state["state"]
+    m = (1 << 64) - 1
+    return (ctypes.c_uint64 * 4)(s["state"] >> 64, s["state"] & m, s["inc"] >> 64, s["inc"] & m)
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not EMU.exists():
+        import build_native
+        build_native.build_emu()
+    lib = ctypes.CDLL(str(EMU))
+    lib.qbx_emu_random_qubo.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int64,
+                                        ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    return lib
+
+
+@pytest.mark.parametrize("seed,n,low,high,dtype", CASES)
+def test_host_build_of_generator_matches_numpy(emu, seed, n, low, high, dtype):
+    ref = RandomQUBO.init_random_instance(seed=seed, N=n, low=low, high=high, dtype=dtype).build_Q_matrix()
+    out = np.empty((n, n), dtype=np.int32 if CODES[dtype] > 0 else np.float64)
+    assert emu.qbx_emu_random_qubo(_state(seed), n, low, high, CODES[dtype], out.ctypes.data) == 0
+    assert np.array_equal(out, ref.astype(out.dtype))
+
+
+@pytest.mark.gpu
+def test_device_generator_matches_numpy(gpu_ctx):
+    for dtype in (np.int16, np.int32, np.float32, np.float64):
+        sel = [c for c in CASES if c[4] is dtype]
+        for low, high in sorted({(c[2], c[3]) for c in sel}):
+            group = [c for c in sel if (c[2], c[3]) == (low, high)]
+            mats = gpu_ctx.random_qubo_read([c[0] for c in group], [c[1] for c in group], low, high, dtype)
+            for (seed, n, _, _, _), m in zip(group, mats):
+                ref = RandomQUBO.init_random_instance(seed=seed, N=n, low=low, high=high, dtype=dtype).build_Q_matrix()
+                assert np.array_equal(m, ref.astype(m.dtype)), (seed, n, low, high, dtype)
+
+
+@pytest.mark.gpu
+def test_device_built_batch_equals_host_built_solves(gpu_ctx):
+    from qubo_lib_amd import engine
+    rng = np.random.default_rng(17)
+    seeds = [int(s) for s in rng.integers(0, 2 ** 31, 48)]
+    ns = [int(n) for n in rng.integers(8, 27, 48)]
+    for dtype in (np.int16, np.float32):
+        res = gpu_ctx.random_qubo_batch(seeds, ns, -100, 100, dtype)
+        for seed, n, r in zip(seeds, ns, res):
+            Q = RandomQUBO.init_random_instance(seed=seed, N=n, dtype=dtype).build_Q_matrix()
+            alone = engine.solve_matrix(Q)
+            assert r.status == 0 and r.state == alone.state, (seed, n, dtype)
+            assert (r.E_int if dtype is np.int16 else r.E) == alone.E
+
+
+@pytest.mark.gpu
+def test_solve_batch_draws_random_qubo_on_device(gpu_ctx):
+    from qubo_lib_amd.batch import solve_batch
+    probs = [RandomQUBO.init_random_instance(seed=100 + i, N=10 + i % 14, dtype=np.int16) for i in range(80)]
+    dev = solve_batch(probs)                       # 80 seeded RandomQUBO: drawn on the device
+    host = solve_batch(probs, draw_on_device=False)
+    for a, b in zip(dev, host):
+        assert a.best_E == b.best_E and list(a.best_X) == list(b.best_X)
+        assert a.ising_offset == b.ising_offset

@@ -126,3 +126,16 @@ def test_tuning_table_files(tmp_path, monkeypatch):
     bad.write_text(json.dumps({"walk_bits": {"i16/single": [-1]}}))
     with pytest.raises(ValueError):
         _native.read_tuning(bad)
+
+
+def test_device_draw_selection():
+    from qubo_lib_amd.batch import _drawable_on_device
+    from qubo_lib_amd.problems import MaxCut
+    r = [RandomQUBO.init_random_instance(seed=i, N=12, dtype=np.int16) for i in range(3)]
+    assert _drawable_on_device(r)
+    assert not _drawable_on_device(r + [RandomQUBO.init_random_instance(seed=9, N=12, dtype=np.int32)])
+    assert not _drawable_on_device(r + [MaxCut.random_gnp(8, 0.5, seed=1)])
+    big = [RandomQUBO.init_random_instance(seed=1, N=12, low=-2 ** 30, high=2 ** 30, dtype=np.int32)]
+    assert not _drawable_on_device(big)            # int32 bound cannot be guaranteed
+    assert _drawable_on_device([RandomQUBO.init_random_instance(seed=1, N=40, dtype=np.float32)])
+    assert not _drawable_on_device([])
