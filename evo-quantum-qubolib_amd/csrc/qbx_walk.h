@@ -249,6 +249,39 @@ QBX_HD void trans_static_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ WJ
     }
 }
 
+// Row M+I of W (entries 0 .. M+I-1: the inner bits and the outer bits below I), loaded one
+// block ahead of the static transition that uses it, so that its memory latency is hidden
+// behind a block walk.
+template <int M>
+struct WRow {
+    int32_t w[M + 4];
+};
+
+template <int M, int I>
+QBX_HD void load_wrow(WRow<M>& r, const int32_t* __restrict__ W) {
+    const int32_t* WJ = W + (M + I) * LDW;
+#pragma unroll
+    for (int k = 0; k < M + I; ++k) r.w[k] = WJ[k];
+}
+
+template <int M, int OMAX, int I>
+QBX_HD void trans_static_row_i32(LaneI32<M, OMAX>& s, const WRow<M>& r, bool up) {
+    const uint32_t field = (uint32_t)s.u[I] + (I > 0 ? (uint32_t)r.w[M + I - 1] : 0u);
+    if (up) {
+        s.E0 = (int32_t)((uint32_t)s.E0 + field);
+#pragma unroll
+        for (int k = 0; k < M; ++k) { s.f[k] += r.w[k]; s.nf[k] -= r.w[k]; }
+#pragma unroll
+        for (int q = 0; q < I; ++q) s.u[q] += r.w[M + q];
+    } else {
+        s.E0 = (int32_t)((uint32_t)s.E0 - field);
+#pragma unroll
+        for (int k = 0; k < M; ++k) { s.f[k] -= r.w[k]; s.nf[k] += r.w[k]; }
+#pragma unroll
+        for (int q = 0; q < I; ++q) s.u[q] -= r.w[M + q];
+    }
+}
+
 // Same for a run-time bit index i >= GROUP_BITS (taken once per group of 2^GROUP_BITS blocks).
 template <int M, int OMAX>
 QBX_HD void trans_dyn_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ WJ, int i, bool up) {
@@ -298,7 +331,9 @@ struct LaneBest {
 
 template <int M, int OMAX, int J>
 QBX_HD bool group_step_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* dl,
-                           uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best) {
+                           uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best,
+                           WRow<M>& row) {
+    static_assert(GROUP_BITS == 4, "WRow holds rows of the outer bits below 4");
     const uint32_t c = c0 + (uint32_t)J;
     if (J > 0) {
         if (c >= nblk) return false;
@@ -306,8 +341,10 @@ QBX_HD bool group_step_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
         // gray(c) bit i = c_i ^ c_{i+1}, and c_i = 1 at the flip
         const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
                                              : (((c0 >> GROUP_BITS) & 1u) == 0u);
-        trans_static_i32<M, OMAX, i>(s, W + (M + i) * LDW, up);
+        trans_static_row_i32<M, OMAX, i>(s, row, up);
     }
+    // the next block's transition row, in flight while this block is walked
+    if (J + 1 < (1 << GROUP_BITS)) load_wrow<M, __builtin_ctz((unsigned)(J + 1))>(row, W);
     const int32_t mn = block_min_i32<M>(s.E0, s.f, s.nf, dl);
     // rare after the first blocks: one compare per block on the common path, the block key
     // only when the block ties or beats the lane's best
@@ -322,7 +359,8 @@ template <int M, int OMAX, int... Js>
 QBX_HD void group_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* dl,
                       uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best,
                       std::integer_sequence<int, Js...>) {
-    (void)(group_step_i32<M, OMAX, Js>(s, W, dl, c0, nblk, SBk, pmaskS, best) && ...);
+    WRow<M> row;
+    (void)(group_step_i32<M, OMAX, Js>(s, W, dl, c0, nblk, SBk, pmaskS, best, row) && ...);
 }
 
 // Lane set-up: E0 and the fields of the lane's first block.  The lane prefix p splits into
@@ -646,6 +684,7 @@ QBX_HD bool group_step_q32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ Wq, 
         constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
         const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
                                              : (((c0 >> GROUP_BITS) & 1u) == 0u);
+        // (no row prefetch here: its registers would cost this kernel its second wave)
         trans_static_i32<M, OMAX, i>(s, Wq + (M + i) * LDW, up);
     }
     const int32_t mn = block_min_i32<M>(s.E0, s.f, s.nf, dlq);
