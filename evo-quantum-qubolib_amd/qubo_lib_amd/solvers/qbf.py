@@ -15,8 +15,9 @@ kwargs (qbf.py:52-59, 429-441):
                      Default B = min(N, 20), the reference CPU default.
   use_mem_GB         if given (and A, B are not), B is derived from it exactly as the
                      reference does (adjust_AB_to_fit_memory, qbf.py:455-466).
-  correction_period  accepted for compatibility; the float path re-anchors in FP64 at
-                     every block of 2^inner_bits states, which subsumes it.
+  correction_period  accepted for compatibility and not needed: the float path filters
+                     in exact int32 arithmetic (no drift to correct) and computes every
+                     energy it compares in FP64 from scratch.
   use_metrics        'time_arr' / 'energy_arr' / 'all' (qbf.py:342-351); see
                      pack_results for what each holds.
   walk_bits, inner_bits  tuning overrides (bits walked per lane, unrolled block bits).
