@@ -3,6 +3,8 @@
   libqbx.so      hipcc, gfx950: kernels + C ABI (include/qbx.h)  -> qubo_lib_amd/libqbx.so
   libqbx_emu.so  g++: host build of the same lane walk, loaded only by the CPU test
                  suite to check the walk's indexing without a GPU -> tests/_build/
+  qbx_c_check    gcc -std=c99: a plain C caller of the C ABI, run by a GPU test
+                 -> tests/_build/
 
 Every call recompiles.  Outputs are git-ignored; they travel to the GPU box with the tree.
 """
@@ -19,6 +21,7 @@ CSRC = HERE / "csrc"
 INCLUDE = REPO / "include"
 LIB_OUT = HERE / "qubo_lib_amd" / "libqbx.so"
 EMU_OUT = REPO / "tests" / "_build" / "libqbx_emu.so"
+C_CHECK_OUT = REPO / "tests" / "_build" / "qbx_c_check"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -56,8 +59,18 @@ def build_emu(out: Path = EMU_OUT) -> Path:
     return out
 
 
+def build_c_check(out: Path = C_CHECK_OUT, lib: Path = LIB_OUT) -> Path:
+    """A plain C99 program linked against libqbx.so (tests/c/qbx_c_check.c): shows that
+    include/qbx.h serves C callers as it is."""
+    out.parent.mkdir(parents=True, exist_ok=True)
+    _run(["gcc", "-std=c99", "-Wall", "-Werror", "-O1", f"-I{INCLUDE}", "-o", str(out),
+          str(REPO / "tests" / "c" / "qbx_c_check.c"), f"-L{lib.parent}", "-lqbx",
+          "-Wl,-rpath,$ORIGIN/" + os.path.relpath(lib.parent, out.parent)])
+    return out
+
+
 def build_all() -> list[Path]:
-    return [build_lib(), build_emu()]
+    return [build_lib(), build_emu(), build_c_check()]
 
 
 if __name__ == "__main__":

@@ -77,3 +77,35 @@ def test_server_rejects_oversized(gpu_ctx):
     with Server(max_n=20) as srv:
         with pytest.raises(ValueError):
             srv.submit(np.zeros((21, 21), dtype=np.int32))
+
+
+def test_server_float_filter_cases_and_tie_width(gpu_ctx):
+    """Float requests that stress the quantized filter, and explicit tie widths, give the
+    answers of plain solves."""
+    rng = np.random.default_rng(21)
+    Qs = []
+    for n in (12, 17, 20):
+        Q = rng.standard_normal((n, n)) * 1e-3
+        Q[0, n - 1] = 1e9                      # small terms quantize to zero
+        Qs.append(Q)
+        Qs.append(rng.standard_normal((n, n)) * 1e-200)
+    Bs = [None, 0, 5, None, 3, 7]
+    with Server(max_n=24, max_inflight=8) as srv:
+        ids = [srv.submit(q, tie_B=b) for q, b in zip(Qs, Bs)]
+        got = drain(srv, len(ids))
+    for q, b, i in zip(Qs, Bs, ids):
+        E, s = oracle.brute_f64(q, b)
+        r = got[i]
+        assert r["status"] == 0 and r["state"] == s, (q.shape, b)
+        assert abs(r["E"] - E) <= 1e-13 * np.abs(q).sum() + 1e-300
+
+
+def test_server_stop_with_requests_pending(gpu_ctx):
+    """Closing the server while requests are queued and running finishes them and returns."""
+    rng = np.random.default_rng(22)
+    srv = Server(max_n=30, max_inflight=4, work_budget=2.0 ** 26)
+    t0 = time.time()
+    for _ in range(40):
+        srv.submit(rng.integers(-50, 50, (26, 26)))
+    srv.close()
+    assert time.time() - t0 < 60
