@@ -65,10 +65,17 @@ def prepare(Q) -> Prepared:
     if Q.dtype == np.bool_:
         Q = Q.astype(np.int64)
     if np.issubdtype(Q.dtype, np.integer):
-        Qo = Q.astype(object)
-        up = np.triu(Qo, 1) + np.triu(Qo.T, 1)          # Q_ij + Q_ji for i < j, exact
-        diag = np.diag(Qo)
-        bound = int(sum(abs(v) for v in diag)) + int(sum(abs(v) for v in up.ravel()))
+        if Q.dtype.itemsize <= 4 or (-2 ** 31 <= int(Q.min()) and int(Q.max()) <= 2 ** 31):
+            # entries within +-2^31: every sum below fits int64 (n <= 56), vectorised
+            Qi = Q.astype(np.int64)
+            up = np.triu(Qi, 1) + np.triu(Qi.T, 1)      # Q_ij + Q_ji for i < j, exact
+            diag = np.diag(Qi)
+            bound = int(np.abs(diag).sum()) + int(np.abs(up).sum())
+        else:
+            Qo = Q.astype(object)                       # arbitrary int64 / uint64: exact in Python ints
+            up = np.triu(Qo, 1) + np.triu(Qo.T, 1)
+            diag = np.diag(Qo)
+            bound = int(sum(abs(v) for v in diag)) + int(sum(abs(v) for v in up.ravel()))
         folded = up + np.diag(diag)
         if bound <= INT32_BOUND:
             return Prepared(n, "i32", True, np.ascontiguousarray(folded.astype(np.int64).astype(np.int32)), bound)

@@ -139,3 +139,23 @@ def test_device_draw_selection():
     assert not _drawable_on_device(big)            # int32 bound cannot be guaranteed
     assert _drawable_on_device([RandomQUBO.init_random_instance(seed=1, N=40, dtype=np.float32)])
     assert not _drawable_on_device([])
+
+
+def test_prepare_fold_matches_exact_python_ints():
+    """The vectorised int64 fold equals the fold in Python ints for every integer dtype."""
+    rng = np.random.default_rng(30)
+    for dtype, lo, hi in [(np.int8, -128, 127), (np.int16, -32768, 32767), (np.int32, -2 ** 31, 2 ** 31 - 1),
+                          (np.uint16, 0, 65535), (np.int64, -2 ** 31, 2 ** 31), (np.int64, -2 ** 40, 2 ** 40)]:
+        for n in (1, 5, 17):
+            Q = rng.integers(lo, hi, (n, n), dtype=np.int64, endpoint=True).astype(dtype)
+            Qo = Q.astype(object)
+            up = np.triu(Qo, 1) + np.triu(Qo.T, 1)
+            bound = int(sum(abs(v) for v in np.diag(Qo))) + int(sum(abs(v) for v in up.ravel()))
+            if bound >= engine.EXACT_F64_BOUND:
+                with pytest.raises(OverflowError):
+                    engine.prepare(Q)
+                continue
+            p = engine.prepare(Q)
+            assert p.bound == bound and p.integer
+            ref = (up + np.diag(np.diag(Qo))).astype(np.int64)
+            assert np.array_equal(p.matrix.astype(np.int64), ref)
