@@ -2,7 +2,7 @@
 //
 // Kernels
 //   qbx_prep         one workgroup per problem: bit-ordered W and d tables, the inner-block
-//                    step constants delta_t, the energy bound (int32 admission), for float
+//                    trie constants c_y, the energy bound (int32 admission), for float
 //                    problems the quantized int32 copies of the tables the filter walks,
 //                    and reset of the per-problem reduction state.
 //                    Replaces the reference's cache building (qbf.py:246-289,

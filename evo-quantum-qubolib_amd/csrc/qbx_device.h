@@ -40,7 +40,8 @@ struct InstDev {
     const void* Q;           // source Q: int32 or double, row-major, variable order
     void* W;                 // n x LDW, bit order: int32 | double followed by int32 Wq
     void* d;                 // 64 entries: int32 | double followed by int32 dq
-    void* delta;             // 2^M entries: int32 | double followed by int32 dlq
+    void* delta;             // 2^M entries: int32 trie constants | FP64 Gray step constants
+                             // followed by the int32 trie constants of the quantized tables
     RedDev* red;
     uint32_t* cand;          // chunks entries
     SlotDev* slots;          // chunks entries
@@ -105,7 +106,7 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             acc += v < 0 ? -v : v;
         }
         for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
-            dl[t] = (int32_t)delta_elem<int32_t, long long>(Q, n, M, t);
+            dl[t] = (int32_t)trie_elem<int32_t, long long>(Q, n, M, t);
         red_i[tid] = acc;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
@@ -170,9 +171,9 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             if (tid < s) red_i[tid] += red_i[tid + s];
             __syncthreads();
         }
-        // step constants of the filter from the quantized couplings (exact int32 sums)
+        // trie constants of the filter from the quantized couplings (exact int32 sums)
         for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
-            dlq[t] = (int32_t)delta_from_table<int32_t, long long>(Wq, M, t);
+            dlq[t] = (int32_t)trie_from_table<int32_t, long long>(Wq, M, t);
         if (tid == 0) {
             I.fbound = fb;
             I.qscale = scale;
@@ -322,7 +323,7 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
         const int n = I.n, L = I.L, B = I.B, H = n - L;
         const int32_t* W = (const int32_t*)I.W;
         const int32_t* d = (const int32_t*)I.d;
-        // inner-step constants: uniform values kept in VGPRs (the lane-dependent zero
+        // inner-block trie constants: uniform values kept in VGPRs (the lane-dependent zero
         // offset keeps them out of the scalar register file)
         int32_t dl[1 << M];
         const int32_t* dsrc = (const int32_t*)I.delta;

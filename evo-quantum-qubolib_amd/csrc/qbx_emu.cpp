@@ -29,7 +29,7 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
         d[a] = d_elem<int32_t, int32_t>(Q, n, a);
         for (int b = 0; b < LDW; ++b) W[(size_t)a * LDW + b] = w_elem<int32_t, int32_t>(Q, n, a, b);
     }
-    for (uint32_t t = 0; t < (1u << M); ++t) dl[t] = delta_elem<int32_t, int32_t>(Q, n, M, t);
+    for (uint32_t t = 0; t < (1u << M); ++t) dl[t] = trie_elem<int32_t, int32_t>(Q, n, M, t);
     const int H = n - L;
     const uint64_t lanes = 1ull << H;
     // the device flow: every lane's best block key, the (E, lane rank, block key) winner
@@ -80,7 +80,7 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
     if (qbound > 0x7ffffffeLL) return -4;
     for (uint32_t t = 0; t < (1u << M); ++t) {
         dld[t] = delta_elem<double, double>(Q, n, M, t);
-        dlq[t] = (int32_t)delta_from_table<int32_t, long long>(Wq.data(), M, t);
+        dlq[t] = (int32_t)trie_from_table<int32_t, long long>(Wq.data(), M, t);
     }
     const double qm = q_margin(n);
     const int H = n - L;

@@ -20,9 +20,11 @@
 // A wave owns 64 consecutive lane ranks lambda; lane prefix p is the top-H-bit state
 // whose ranks start at lambda << L, so lane ranks are ordered like states.  The lane
 // walks its 2^L states as 2^O blocks of 2^M states:
-//   * inner block: a compile-time unrolled Gray walk over the M low bits.  Each step is
-//       E += (+-f_k) + delta_t      (one v_add3_u32; f_k / -f_k per lane, delta_t uniform)
+//   * inner block: a compile-time unrolled walk of the M low bits as a binary trie (each
+//     state is its parent with the lowest set bit cleared, plus that bit):
+//       E(y) = E(parent) + f_k + c_y   (one v_add3_u32; f_k per lane, c_y uniform)
 //     and block minima are taken two states at a time (v_min3_i32): ~1.5 VALU ops/state.
+//     Only +f_k is ever added, so no negated fields are kept or updated.
 //   * block transition: the outer bits advance in Gray order; flipping outer bit J
 //     changes E0 (energy of the block base) by +-(u_J + W[J][J-1]) because, at that
 //     moment, the only walked bit below J that is set is J-1.  Fields f_k of the inner
@@ -125,7 +127,8 @@ QBX_HD Acc d_elem(const T* Q, int n, int a) {
     return Acc(Q[(int64_t)i * n + i]);
 }
 // inner-step constant delta_t (t = 1 .. 2^M - 1): the pairwise part of the energy change
-// when step t of the inner Gray walk flips inner bit k = ctz(t).
+// when step t of a Gray walk over the inner bits flips inner bit k = ctz(t) (the FP64
+// block re-walk of float problems walks its blocks in Gray order).
 template <class T, class Acc>
 QBX_HD Acc delta_elem(const T* Q, int n, int M, uint32_t t) {
     if (t == 0 || t >= (1u << M)) return Acc(0);
@@ -139,12 +142,37 @@ QBX_HD Acc delta_elem(const T* Q, int n, int M, uint32_t t) {
     return up ? s : Acc(0) - s;
 }
 
+// Inner-block trie: every inner state y != 0 is its parent p = y & (y - 1) (lowest set bit
+// cleared) plus bit k = ctz(y), so E(y) = E(p) + f_k + c_y with the per-problem constant
+//     c_y = sum over set bits l of p of W[k][l]        (y = 1 .. 2^M - 1).
+// Walking the block as this trie only ever adds a field (never -f_k as a Gray walk does).
+template <class T, class Acc>
+QBX_HD Acc trie_elem(const T* Q, int n, int M, uint32_t y) {
+    if (y == 0 || y >= (1u << M)) return Acc(0);
+    const int k = ctz32(y);
+    const uint32_t p = y & (y - 1u);
+    Acc s = Acc(0);
+    for (int l = 0; l < M; ++l)
+        if ((p >> l) & 1u) s += w_elem<T, Acc>(Q, n, k, l);
+    return s;
+}
+// the same from a bit-ordered table (row k, column l = W[k][l])
+template <class T, class Acc>
+QBX_HD Acc trie_from_table(const T* W, int M, uint32_t y) {
+    if (y == 0 || y >= (1u << M)) return Acc(0);
+    const int k = ctz32(y);
+    const uint32_t p = y & (y - 1u);
+    Acc s = Acc(0);
+    for (int l = 0; l < M; ++l)
+        if ((p >> l) & 1u) s += Acc(W[k * LDW + l]);
+    return s;
+}
+
 // ------------------------------------------------------------------ int32 lane walk
 template <int M, int OMAX>
 struct LaneI32 {
     int32_t E0;          // energy of the block base (inner bits = 0)
     int32_t f[M];        // field of inner bit k from every non-inner bit
-    int32_t nf[M];       // -f[k]
     int32_t u[OMAX];     // upper field of outer bit M+q: d + sum over set bits above it
 };
 
@@ -169,58 +197,83 @@ QBX_HD int32_t min3_i32(int32_t mn, int32_t a, int32_t b) {
 #define QBX_OPAQUE(x) ((void)0)
 #endif
 
-// Minimum energy over one inner block.
-template <int M>
-QBX_HD int32_t block_min_i32(int32_t E0, const int32_t* f, const int32_t* nf, const int32_t* dl) {
-    uint32_t E = (uint32_t)E0;
-    int32_t mn = E0;
-    int32_t odd = E0;
-#pragma unroll
-    for (int t = 1; t < (1 << M); ++t) {
-        const int k = __builtin_ctz((unsigned)t);
-        const bool up = ((((unsigned)t ^ ((unsigned)t >> 1)) >> k) & 1u) != 0u;
-        E = E + (uint32_t)(up ? f[k] : nf[k]) + (uint32_t)dl[t];
-        if (t & 1) odd = (int32_t)E;
-        else mn = min3_i32(mn, odd, (int32_t)E);
+// Minimum energy over one inner block, walked as the trie in depth-first order: each even
+// node P is paired with its leaf P | 1 (one v_add3_u32 each, one v_min3_i32 per pair), then
+// P's even children P | 2^k (k < ctz(P), or any k for P = 0) are visited.  At most M - 1
+// energies are live on the way down.
+QBX_HD constexpr int ctz_c(unsigned v) { return v ? __builtin_ctz(v) : 32; }
+
+template <int M, unsigned P>
+QBX_HD void trie_even(uint32_t EP, const int32_t* f, const int32_t* cc, int32_t& mn);
+
+template <int M, unsigned P, int K>
+QBX_HD void trie_children(uint32_t EP, const int32_t* f, const int32_t* cc, int32_t& mn) {
+    if constexpr (K < M && K < ctz_c(P)) {
+        constexpr unsigned C = P | (1u << K);
+        trie_even<M, C>(EP + (uint32_t)f[K] + (uint32_t)cc[C], f, cc, mn);
+        trie_children<M, P, K + 1>(EP, f, cc, mn);
     }
-    return min3_i32(mn, odd, odd);   // t = 2^M - 1 is odd and has no partner
+}
+
+template <int M, unsigned P>
+QBX_HD void trie_even(uint32_t EP, const int32_t* f, const int32_t* cc, int32_t& mn) {
+    const uint32_t Eodd = EP + (uint32_t)f[0] + (uint32_t)cc[P | 1u];
+    mn = min3_i32(mn, (int32_t)EP, (int32_t)Eodd);
+    trie_children<M, P, 1>(EP, f, cc, mn);
+}
+
+template <int M>
+QBX_HD int32_t block_min_i32(int32_t E0, const int32_t* f, const int32_t* cc) {
+    int32_t mn = E0;
+    trie_even<M, 0u>((uint32_t)E0, f, cc, mn);
+    return mn;
 }
 
 // Lowest inner rank among the states of the block whose energy equals mn.  The walk only
-// records which steps hit mn (a 2^M-bit mask); ranks are computed afterwards for the set
-// bits alone, so no per-step rank arithmetic exists for the compiler to hoist.
+// records which states hit mn (a 2^M-bit mask, bit y for inner state y); ranks are
+// computed afterwards for the set bits alone, so no per-state rank arithmetic exists for
+// the compiler to hoist.
+template <int M, unsigned P>
+QBX_HD void trie_even_mask(uint32_t EP, const int32_t* f, const int32_t* cc, int32_t mn, uint32_t* mask);
+
+template <int M, unsigned P, int K>
+QBX_HD void trie_children_mask(uint32_t EP, const int32_t* f, const int32_t* cc, int32_t mn, uint32_t* mask) {
+    if constexpr (K < M && K < ctz_c(P)) {
+        constexpr unsigned C = P | (1u << K);
+        trie_even_mask<M, C>(EP + (uint32_t)f[K] + (uint32_t)cc[C], f, cc, mn, mask);
+        trie_children_mask<M, P, K + 1>(EP, f, cc, mn, mask);
+    }
+}
+
+template <int M, unsigned P>
+QBX_HD void trie_even_mask(uint32_t EP, const int32_t* f, const int32_t* cc, int32_t mn, uint32_t* mask) {
+    const uint32_t Eodd = EP + (uint32_t)f[0] + (uint32_t)cc[P | 1u];
+    if ((int32_t)EP == mn) mask[P >> 5] |= 1u << (P & 31u);
+    if ((int32_t)Eodd == mn) mask[(P | 1u) >> 5] |= 1u << ((P | 1u) & 31u);
+    trie_children_mask<M, P, 1>(EP, f, cc, mn, mask);
+}
+
 template <int M>
-QBX_HD uint32_t block_argmin_i32(int32_t E0, const int32_t* f, const int32_t* nf, const int32_t* dl,
-                                 int32_t mn, int B, int par) {
-    int32_t fo[M], nfo[M];
+QBX_HD uint32_t block_argmin_i32(int32_t E0, const int32_t* f, const int32_t* cc, int32_t mn, int B, int par) {
+    int32_t fo[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         fo[k] = f[k];
-        nfo[k] = nf[k];
         QBX_OPAQUE(fo[k]);
-        QBX_OPAQUE(nfo[k]);
     }
     QBX_OPAQUE(E0);
     constexpr int NW = ((1 << M) + 31) / 32;
     uint32_t mask[NW];
 #pragma unroll
     for (int w = 0; w < NW; ++w) mask[w] = 0u;
-    uint32_t E = (uint32_t)E0;
-    if (E0 == mn) mask[0] = 1u;
-#pragma unroll
-    for (int t = 1; t < (1 << M); ++t) {
-        const int k = __builtin_ctz((unsigned)t);
-        const bool up = ((((unsigned)t ^ ((unsigned)t >> 1)) >> k) & 1u) != 0u;
-        E = E + (uint32_t)(up ? fo[k] : nfo[k]) + (uint32_t)dl[t];
-        if ((int32_t)E == mn) mask[t >> 5] |= 1u << (t & 31);
-    }
+    trie_even_mask<M, 0u>((uint32_t)E0, fo, cc, mn, mask);
     uint32_t best = 0xffffffffu;
     for (int w = 0; w < NW; ++w) {
         uint32_t m = mask[w];
         while (m) {
-            const uint32_t t = (uint32_t)(w * 32 + __builtin_ctz(m));
+            const uint32_t y = (uint32_t)(w * 32 + __builtin_ctz(m));
             m &= m - 1u;
-            const uint32_t r = rank_lo<M>(t ^ (t >> 1), B, par);
+            const uint32_t r = rank_lo<M>(y, B, par);
             best = r < best ? r : best;
         }
     }
@@ -237,13 +290,13 @@ QBX_HD void trans_static_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ WJ
     if (up) {
         s.E0 = (int32_t)((uint32_t)s.E0 + field);
 #pragma unroll
-        for (int k = 0; k < M; ++k) { s.f[k] += WJ[k]; s.nf[k] -= WJ[k]; }
+        for (int k = 0; k < M; ++k) s.f[k] += WJ[k];
 #pragma unroll
         for (int q = 0; q < I; ++q) s.u[q] += WJ[M + q];
     } else {
         s.E0 = (int32_t)((uint32_t)s.E0 - field);
 #pragma unroll
-        for (int k = 0; k < M; ++k) { s.f[k] -= WJ[k]; s.nf[k] += WJ[k]; }
+        for (int k = 0; k < M; ++k) s.f[k] -= WJ[k];
 #pragma unroll
         for (int q = 0; q < I; ++q) s.u[q] -= WJ[M + q];
     }
@@ -270,13 +323,13 @@ QBX_HD void trans_static_row_i32(LaneI32<M, OMAX>& s, const WRow<M>& r, bool up)
     if (up) {
         s.E0 = (int32_t)((uint32_t)s.E0 + field);
 #pragma unroll
-        for (int k = 0; k < M; ++k) { s.f[k] += r.w[k]; s.nf[k] -= r.w[k]; }
+        for (int k = 0; k < M; ++k) s.f[k] += r.w[k];
 #pragma unroll
         for (int q = 0; q < I; ++q) s.u[q] += r.w[M + q];
     } else {
         s.E0 = (int32_t)((uint32_t)s.E0 - field);
 #pragma unroll
-        for (int k = 0; k < M; ++k) { s.f[k] -= r.w[k]; s.nf[k] += r.w[k]; }
+        for (int k = 0; k < M; ++k) s.f[k] -= r.w[k];
 #pragma unroll
         for (int q = 0; q < I; ++q) s.u[q] -= r.w[M + q];
     }
@@ -293,7 +346,7 @@ QBX_HD void trans_dyn_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ WJ, i
     const int32_t sg = up ? 1 : -1;
     s.E0 = (int32_t)((uint32_t)s.E0 + (uint32_t)sg * field);
 #pragma unroll
-    for (int k = 0; k < M; ++k) { const int32_t w = sg * WJ[k]; s.f[k] += w; s.nf[k] -= w; }
+    for (int k = 0; k < M; ++k) s.f[k] += sg * WJ[k];
 #pragma unroll
     for (int q = 0; q < OMAX; ++q) if (q < i) s.u[q] += sg * WJ[M + q];
 }
@@ -345,7 +398,7 @@ QBX_HD bool group_step_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
     }
     // the next block's transition row, in flight while this block is walked
     if (J + 1 < (1 << GROUP_BITS)) load_wrow<M, __builtin_ctz((unsigned)(J + 1))>(row, W);
-    const int32_t mn = block_min_i32<M>(s.E0, s.f, s.nf, dl);
+    const int32_t mn = block_min_i32<M>(s.E0, s.f, dl);
     // rare after the first blocks: one compare per block on the common path, the block key
     // only when the block ties or beats the lane's best
     if (__builtin_expect(mn <= best.E, 0)) {
@@ -425,8 +478,6 @@ QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
         for (int q = 0; q < OMAX; ++q) s.u[q] += (int32_t)((uint32_t)Wa[M + q] & mi);
     }
     s.E0 = (int32_t)E0;
-#pragma unroll
-    for (int k = 0; k < M; ++k) s.nf[k] = -s.f[k];
 }
 
 // Lane key: (E, K) packed as one signed 64-bit value, E in the high word.  Ordering lane
@@ -436,7 +487,7 @@ QBX_HD uint32_t key_K(int64_t key) { return (uint32_t)key; }
 constexpr int64_t KEY_NONE = 0x7fffffffffffffffll;
 
 // Walk one lane's 2^L states and return the key of its best block.
-// W: bit-ordered, LDW columns; d: n; dl: delta_t for t < 2^M.
+// W: bit-ordered, LDW columns; d: n; dl: the trie constants c_y for y < 2^M.
 template <int M, int OMAX>
 QBX_HD int64_t lane_best_block_i32(const int32_t* __restrict__ W, const int32_t* __restrict__ d,
                                    int n, int L, int B, uint64_t lam, const int32_t* dl) {
@@ -480,11 +531,9 @@ QBX_HD uint32_t block_rank_lo_i32(const int32_t* __restrict__ W, const int32_t* 
     const uint32_t gc = c ^ (c >> 1);
     const uint64_t hs = (p << O) | (uint64_t)gc;
     int32_t bE0;
-    int32_t bf[M], bnf[M];
+    int32_t bf[M];
     block_state_i32<M>(W, d, n, hs, bE0, bf);
-#pragma unroll
-    for (int kk = 0; kk < M; ++kk) bnf[kk] = -bf[kk];
-    return block_argmin_i32<M>(bE0, bf, bnf, dl, E, B, parp ^ parity32(gc));
+    return block_argmin_i32<M>(bE0, bf, dl, E, B, parp ^ parity32(gc));
 }
 
 // Rank of the best state of block K of lane lam.
@@ -538,20 +587,6 @@ QBX_HD int32_t q_threshold(double b, double scale, double qm) {
     if (t <= -2147483648.0) return (int32_t)0x80000000;
     return (int32_t)t;
 }
-// inner-step constant delta_t from a bit-ordered table (row k, column l = W[k][l])
-template <class T, class Acc>
-QBX_HD Acc delta_from_table(const T* W, int M, uint32_t t) {
-    if (t == 0 || t >= (1u << M)) return Acc(0);
-    const int k = ctz32(t);
-    const uint32_t g = t ^ (t >> 1);
-    const uint32_t y = (t - 1) ^ ((t - 1) >> 1);
-    const bool up = ((g >> k) & 1u) != 0u;
-    Acc s = Acc(0);
-    for (int l = 0; l < M; ++l)
-        if (l != k && ((y >> l) & 1u)) s += Acc(W[k * LDW + l]);
-    return up ? s : Acc(0) - s;
-}
-
 // E0 and inner fields of an arbitrary block, from scratch, in FP64 (hs: bits above the block).
 template <int M>
 QBX_HD void block_state_f64(const double* __restrict__ W, const double* __restrict__ d, int n,
@@ -687,7 +722,7 @@ QBX_HD bool group_step_q32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ Wq, 
         // (no row prefetch here: its registers would cost this kernel its second wave)
         trans_static_i32<M, OMAX, i>(s, Wq + (M + i) * LDW, up);
     }
-    const int32_t mn = block_min_i32<M>(s.E0, s.f, s.nf, dlq);
+    const int32_t mn = block_min_i32<M>(s.E0, s.f, dlq);
     if (mn <= q.thr) {           // rare after the first blocks: record the block
         q.laneMin = mn < q.laneMin ? mn : q.laneMin;
         q_update_thr(q);
