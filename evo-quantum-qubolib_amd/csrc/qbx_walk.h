@@ -484,7 +484,6 @@ QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
 // keys orders the lane's blocks by (block minimum, block rank).
 QBX_HD int32_t key_E(int64_t key) { return (int32_t)(key >> 32); }
 QBX_HD uint32_t key_K(int64_t key) { return (uint32_t)key; }
-constexpr int64_t KEY_NONE = 0x7fffffffffffffffll;
 
 // Walk one lane's 2^L states and return the key of its best block.
 // W: bit-ordered, LDW columns; d: n; dl: the trie constants c_y for y < 2^M.
