@@ -1,0 +1,40 @@
+"""Randomised parity sweep: random kinds, sizes, tie widths, inner-block variants and walk
+widths through the int32 kernel and the float kernel (called directly, so integer and
+perturbed matrices also exercise the quantized filter), and mixed batches, all against
+the oracle's brute force.  (A 10 000-check run of the same sweep passed as well.)"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import KINDS, random_instance
+from qubo_lib_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(7, 13), (6, 14), (4, 16), (1, 19)]
+
+
+def test_random_sweep(gpu_ctx):
+    rng = np.random.default_rng(2024)
+    for trial in range(800):
+        n = int(rng.integers(1, 25))
+        Q = random_instance(rng, n, KINDS[trial % len(KINDS)])
+        B = int(rng.integers(0, n + 1)) if trial % 3 else None
+        cands = [v for v in VARIANTS if v[0] <= n]
+        M, OMAX = cands[trial % len(cands)]
+        L = int(rng.integers(M, min(n, M + OMAX) + 1))
+        ref = oracle.brute_i64(Q, B)
+        got = engine.solve_matrix(engine.prepare(Q), tie_B=B, walk_bits=L, inner_bits=M)
+        assert (got.E, got.state) == ref, ("int", trial, n, B, L, M)
+        Qf = Q.astype(np.float64) + (rng.standard_normal(Q.shape) * 1e-3 if trial % 2 else 0.0)
+        Ef, sf = oracle.brute_f64(Qf, B)
+        r = gpu_ctx.solve_f64(Qf, tie_B=B, walk_bits=L, inner_bits=M)
+        assert r.state == sf and abs(r.E - Ef) <= 1e-12 * max(1.0, np.abs(Qf).sum()), ("f64", trial, n, B, L, M)
+
+
+def test_random_batches(gpu_ctx):
+    rng = np.random.default_rng(2025)
+    for rep in range(5):
+        Qs = [random_instance(rng, int(rng.integers(1, 21)), KINDS[i % len(KINDS)]) for i in range(200)]
+        for Q, r in zip(Qs, engine.solve_matrices(Qs)):
+            assert (r.E, r.state) == oracle.brute_i64(Q), (rep, Q.shape)
