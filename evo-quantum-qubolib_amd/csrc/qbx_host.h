@@ -43,6 +43,7 @@ constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 constexpr int MAX_N = 56;
 constexpr int MAX_LANE_BITS = 37;   // tiles = 2^(H-6) < 2^31
+constexpr uint32_t MAX_CHUNKS = 1u << 22;
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -133,6 +134,9 @@ static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mo
     const uint32_t cap = p.tiles / min_chunks;
     if (tpc > cap) tpc = cap;
     if (tpc < 1u) tpc = 1u;
+    // at most MAX_CHUNKS chunks (per-chunk result slots and atomics stay bounded for n > 44)
+    const uint32_t floor_tpc = (uint32_t)(((uint64_t)p.tiles + MAX_CHUNKS - 1) / MAX_CHUNKS);
+    if (tpc < floor_tpc) tpc = floor_tpc;
     p.tpc = tpc;
     p.chunks = (p.tiles + tpc - 1) / tpc;
     return QBX_OK;

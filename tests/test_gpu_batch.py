@@ -116,3 +116,14 @@ def test_walk_table_changes_tiling_not_answers(gpu_ctx):
     finally:
         gpu_ctx.set_walk_table("i32", "single", [])
         gpu_ctx.load_tuning()
+
+
+def test_chunk_count_cap_keeps_answers(gpu_ctx):
+    """A tiling with more than 2^22 tiles groups several tiles per chunk (the per-chunk
+    result slots stay bounded); the answer equals the default tiling's."""
+    from qubo_lib_amd.problems.generators import symmetric_int_qubo
+    Q = symmetric_int_qubo(36, 5).astype(np.int32)
+    ref = gpu_ctx.solve_i32(Q)
+    r = gpu_ctx.solve_i32(Q, walk_bits=7)          # 2^29 lanes = 2^23 tiles
+    assert r.tiles == 1 << 23
+    assert (r.E_int, r.state) == (ref.E_int, ref.state)
