@@ -197,6 +197,8 @@ def exhaustive(args, ctx, devinfo):
                            "note": "wall clock around the K solve calls: staging Q from host memory, "
                                    "launches, result read-back and synchronisation included"},
     }
+    if args.shard_parts > 1:
+        line["shard"] = shard_shares(prep, args.shard_parts, r, kw)
     if args.cpu_baseline:
         log("timing the CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline(Q)
@@ -206,6 +208,21 @@ def exhaustive(args, ctx, devinfo):
     else:
         line["cpu_baseline"] = None
     return line
+
+
+def shard_shares(prep, parts: int, full, kw) -> dict:
+    """One GPU's share of the same solve spread over `parts` GPUs: each share solved once on
+    this GPU (qbx_solve_part), with its device time; the merged answer must be the full one."""
+    from qubo_lib_amd import engine
+    log(f"solving the {parts} shares ...")
+    res = [engine.solve_part(prep, k, parts, **kw) for k in range(parts)]
+    merged = engine.merge_parts(res)
+    ms = [r.info["kernel_ms"] for r in res]
+    return {"parts": parts, "share_kernel_ms": ms, "max_share_kernel_ms": max(ms),
+            "share_states": [r.info["states"] for r in res],
+            "share_rate": {"value": min(r.info["states"] / (r.info["kernel_ms"] * 1e-3) for r in res),
+                           "unit": "states/s", "note": "slowest share: its states / its device time"},
+            "merged_matches_full": (merged.E, merged.state) == (full.E, full.state)}
 
 
 def batch(args, ctx, devinfo):
@@ -432,10 +449,13 @@ def main():
     ap.add_argument("--no-batch-plain", dest="batch_plain", action="store_false",
                     help="batch workload: skip the one-launch-per-problem comparison (for profiling)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--shard-parts", type=int, default=8,
+                    help="exhaustive workloads: also solve the problem as this many shares (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the prefix space would shard across GPUs with one 8-byte min-reduction per problem)",
+              "(one GPU's share of a sharded solve is measured by --shard-parts; the shares' "
+              "16-byte (E, rank) minima merge on the host)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "copy":

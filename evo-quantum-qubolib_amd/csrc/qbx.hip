@@ -56,6 +56,8 @@ struct Layout {
     std::vector<Group> groups;   // one persistent launch per variant
     size_t off_inst = 0, off_ranges = 0, stage_bytes = 0, off_red = 0, off_res = 0, off_ticket = 0;
     size_t total_bytes = 0;
+    // one part of a single problem's chunks (rank-space shard), or all of them
+    uint32_t part_begin = 0, part_count = 0;   // part_count 0: every chunk
     // matrices generated on the device (RandomQUBO): PCG64 states staged instead of Q
     bool gen = false;
     int gen_dtype = 0;
@@ -171,7 +173,7 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
             lay.groups.push_back({plans[i].variant, r, 0, 0});
         Group& G = lay.groups.back();
         G.cnt++;
-        G.total += plans[i].chunks;
+        G.total += lay.part_count ? lay.part_count : plans[i].chunks;
     }
     if ((int)lay.groups.size() > MAX_GROUPS) return set_err(QBX_EINVAL, "too many kernel variants in one batch");
     return QBX_OK;
@@ -187,6 +189,7 @@ void stage_descriptors(const Layout& lay, char* h, char* dv) {
         memset(&I, 0, sizeof I);
         I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = lay.kind; I.status = 0; I.tiles = p.tiles;
         I.tpc = p.tpc; I.chunks = p.chunks;
+        I.expect = lay.part_count ? lay.part_count : p.chunks;
         I.Q = dv + p.off_Q;
         I.W = dv + p.off_W;
         I.d = dv + p.off_d;
@@ -207,8 +210,9 @@ void stage_descriptors(const Layout& lay, char* h, char* dv) {
             ++next_group;
         }
         TileRange& R = hr[r];
-        R.inst = (uint32_t)i; R.c_begin = 0; R.c_count = lay.plans[i].chunks; R.pad = 0; R.prefix = prefix;
-        prefix += lay.plans[i].chunks;
+        const uint32_t cnt = lay.part_count ? lay.part_count : lay.plans[i].chunks;
+        R.inst = (uint32_t)i; R.c_begin = lay.part_begin; R.c_count = cnt; R.pad = 0; R.prefix = prefix;
+        prefix += cnt;
     }
 }
 
@@ -289,12 +293,23 @@ int collect(const Layout& lay, const ResultDev* hres, float ms, qbx_result_t* ou
 
 template <class T>
 int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* offsets,
-              const int32_t* ns, const qbx_opts_t* opts, qbx_result_t* out, bool single) {
+              const int32_t* ns, const qbx_opts_t* opts, qbx_result_t* out, bool single,
+              int part = 0, int parts = 1) {
     if (!c) return set_err(QBX_EINVAL, "null context");
     if (!Qpacked || !offsets || !ns || !out) return set_err(QBX_EINVAL, "null argument");
     Layout lay;
     int rc = make_layout(c, kind, count, ns, opts, single, lay);
     if (rc) return rc;
+    if (parts != 1) {   // one contiguous range of the (single) problem's chunks
+        const uint32_t chunks = lay.plans[0].chunks;
+        if (count != 1 || parts < 1 || part < 0 || part >= parts || (uint32_t)parts > chunks)
+            return set_err(QBX_EINVAL, "part %d of %d: need 0 <= part < parts <= %u chunks", part, parts, chunks);
+        const uint32_t b = (uint32_t)((uint64_t)chunks * (uint64_t)part / (uint64_t)parts);
+        const uint32_t e = (uint32_t)((uint64_t)chunks * (uint64_t)(part + 1) / (uint64_t)parts);
+        lay.part_begin = b;
+        lay.part_count = e - b;
+        lay.groups[0].total = e - b;
+    }
     rc = ensure_capacity(c, lay.total_bytes, std::max(lay.stage_bytes, sizeof(ResultDev) * (size_t)count));
     if (rc) return rc;
     char* h = c->hbuf;
@@ -308,7 +323,15 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    return collect(lay, (const ResultDev*)h, ms, out);
+    rc = collect(lay, (const ResultDev*)h, ms, out);
+    if (lay.part_count) {   // states of this part only
+        const Plan& p = lay.plans[0];
+        const uint64_t t0 = (uint64_t)lay.part_begin * p.tpc;
+        const uint64_t t1 = std::min<uint64_t>((uint64_t)(lay.part_begin + lay.part_count) * p.tpc, p.tiles);
+        out[0].tiles = t1 - t0;
+        out[0].states = (uint64_t)ldexp((double)(t1 - t0) / (double)p.tiles, p.n);
+    }
+    return rc;
 }
 
 // Device-generated RandomQUBO batches: the layout, the staged PCG64 states and (for
@@ -568,6 +591,20 @@ int qbx_random_qubo_read(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const 
         dst += bytes;
     }
     return QBX_OK;
+}
+
+int qbx_solve_part_i32(qbx_ctx_t* c, const int32_t* Q, int n, const qbx_opts_t* opts, int part, int parts,
+                       qbx_result_t* out) {
+    const int64_t off = 0;
+    const int32_t nn = n;
+    return run_batch<int32_t>(c, KIND_I32, 1, Q, &off, &nn, opts, out, true, part, parts);
+}
+
+int qbx_solve_part_f64(qbx_ctx_t* c, const double* Q, int n, const qbx_opts_t* opts, int part, int parts,
+                       qbx_result_t* out) {
+    const int64_t off = 0;
+    const int32_t nn = n;
+    return run_batch<double>(c, KIND_F64, 1, Q, &off, &nn, opts, out, true, part, parts);
 }
 
 int qbx_batch_i32(qbx_ctx_t* c, int count, const int32_t* Qpacked, const int64_t* offsets,

@@ -36,7 +36,8 @@ struct InstDev {
     uint32_t tiles;          // 64-lane tiles (lane ranks tile*64 .. tile*64+63)
     uint32_t tpc;            // tiles per chunk: one ticket claims one chunk
     uint32_t chunks;         // ceil(tiles / tpc); reduction slots are per chunk
-    uint32_t pad0;
+    uint32_t expect;         // chunks committed by the launch(es) of this problem: all of
+                             // them, or one part's (the last commit finalizes)
     const void* Q;           // source Q: int32 or double, row-major, variable order
     void* W;                 // n x LDW, bit order: int32 | double followed by int32 Wq
     void* d;                 // 64 entries: int32 | double followed by int32 dq
@@ -274,7 +275,7 @@ __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uin
         }
         __threadfence();
         const uint32_t done = atomicAdd(&I.red->done, 1u);
-        last = (done == I.chunks - 1u) ? 1 : 0;
+        last = (done == I.expect - 1u) ? 1 : 0;
     }
     last = __shfl(last, 0);
     if (last) finalize_problem(I, lane);

@@ -143,6 +143,10 @@ _SIGNATURES = {
     "qbx_device_info": (ctypes.c_int, [ctypes.c_void_p, _P(DevInfo)]),
     "qbx_solve_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts), _P(Result)]),
     "qbx_solve_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_solve_part_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts), ctypes.c_int,
+                                          ctypes.c_int, _P(Result)]),
+    "qbx_solve_part_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts), ctypes.c_int,
+                                          ctypes.c_int, _P(Result)]),
     "qbx_batch_i32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_int32), _P(ctypes.c_int64),
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
     "qbx_batch_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
@@ -327,6 +331,20 @@ class Context:
         with self.lock:
             rc = self.lib.qbx_solve_f64(self._h, Q.ctypes.data_as(_P(ctypes.c_double)), n,
                                         ctypes.byref(o), ctypes.byref(out))
+        _check(self.lib, rc)
+        return out
+
+    def solve_part(self, Q: np.ndarray, part: int, parts: int, kind: str = "i32", **kw) -> Result:
+        """(E, rank) minimum over part `part` of `parts` equal ranges of the solve's chunks
+        (one GPU's share of a solve spread over `parts` GPUs)."""
+        dtype, fn, ptr = ((np.int32, self.lib.qbx_solve_part_i32, ctypes.c_int32) if kind == "i32"
+                          else (np.float64, self.lib.qbx_solve_part_f64, ctypes.c_double))
+        Q = np.ascontiguousarray(Q, dtype=dtype)
+        out = Result()
+        o = _opts(**kw)
+        with self.lock:
+            rc = fn(self._h, Q.ctypes.data_as(_P(ptr)), Q.shape[0], ctypes.byref(o), int(part), int(parts),
+                    ctypes.byref(out))
         _check(self.lib, rc)
         return out
 

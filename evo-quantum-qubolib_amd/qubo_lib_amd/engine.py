@@ -120,6 +120,35 @@ def solve_matrix(Q, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None
     return _finish(p, r)
 
 
+def solve_part(Q, part: int, parts: int, device: int = 0, tie_B=None, walk_bits=None,
+               inner_bits=None) -> RawResult:
+    """The (E, rank) minimum over one of `parts` equal shares of the solve's state space (one
+    GPU's share of a solve spread over `parts` GPUs); merge_parts of all shares is
+    solve_matrix's answer.  The share boundaries follow the solve's tiling, so every share
+    must be solved with the same walk_bits / inner_bits."""
+    p = Q if isinstance(Q, Prepared) else prepare(Q)
+    if tie_B is not None and not (0 <= int(tie_B) <= p.n):
+        raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
+    ctx = _native.get_context(device)
+    r = ctx.solve_part(p.matrix, part, parts, kind=p.kind, tie_B=tie_B, walk_bits=walk_bits,
+                       inner_bits=inner_bits)
+    return _finish(p, r)
+
+
+def merge_parts(parts) -> RawResult:
+    """Lexicographic minimum of (E, rank) over share results: the tie rule of the whole solve
+    (an equal energy goes to the lower rank, whichever share found it)."""
+    parts = list(parts)
+    if not parts:
+        raise ValueError("no parts to merge")
+    best = min(parts, key=lambda r: (r.E, r.rank))
+    info = dict(best.info, states=sum(r.info.get("states", 0) for r in parts),
+                tiles=sum(r.info.get("tiles", 0) for r in parts), parts=len(parts),
+                kernel_ms=max(r.info.get("kernel_ms", 0.0) for r in parts))
+    return RawResult(E=best.E, state=best.state, rank=best.rank, n=best.n, tie_B=best.tie_B, kind=best.kind,
+                     info=info)
+
+
 def raw_from_native(r: _native.Result, integer: bool) -> RawResult:
     """RawResult of a problem whose matrix never reached the host (device-drawn batches):
     integer problems on the int32 path, float problems on the FP64 path."""

@@ -96,6 +96,18 @@ int qbx_solve_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opt
 int qbx_solve_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts,
                   qbx_result_t *out);
 
+/* One part of a solve: the states in part `part` of `parts` equal, contiguous ranges of the
+ * solve's chunks (chunks are runs of tiles, tiles runs of the state's leading bits; needs
+ * parts <= chunks).  out holds the (E, rank) minimum over that part (out.states, out.tiles:
+ * the part's share); the lexicographic minimum of (E, rank) over all parts is exactly
+ * qbx_solve_*'s answer.  This is one GPU's share of a solve spread over `parts` GPUs, whose
+ * merge is a minimum over `parts` 16-byte keys.  Counterpart of the reference's bit-fixing
+ * split into independent tasks (parallel.py:145-157), which also splits by leading bits. */
+int qbx_solve_part_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opts, int part, int parts,
+                       qbx_result_t *out);
+int qbx_solve_part_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts, int part, int parts,
+                       qbx_result_t *out);
+
 /* ------------------------------------------------------------------ batches
  * `count` independent problems solved in one persistent launch (plus one table-building
  * launch).  Q_i is stored row-major at Qpacked + offsets[i] with n = ns[i].  Replaces a
