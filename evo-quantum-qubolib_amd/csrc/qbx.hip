@@ -271,6 +271,7 @@ int collect(const Layout& lay, const ResultDev* hres, float ms, qbx_result_t* ou
         R.tiles = p.tiles;
         R.kernel_ms = ms;
         R.status = hres[i].status;
+        R.path = hres[i].path;
         R.rank = hres[i].rank;
         R.state = hres[i].state;
         if (lay.kind == KIND_I32) {

@@ -24,10 +24,11 @@ struct ResultDev {
     uint64_t rank;
     uint64_t state;
     int32_t status;
-    int32_t pad;
+    int32_t path;            // QBX_PATH_*
 };
 
 enum { KIND_I32 = 0, KIND_F64 = 1 };
+static_assert(PAIR_BOUND == QBX_PAIR_BOUND, "pair walk bound of the header and the walk differ");
 
 struct InstDev {
     int32_t n, B, L, M;
@@ -39,10 +40,12 @@ struct InstDev {
     uint32_t expect;         // chunks committed by the launch(es) of this problem: all of
                              // them, or one part's (the last commit finalizes)
     const void* Q;           // source Q: int32 or double, row-major, variable order
-    void* W;                 // n x LDW, bit order: int32 | double followed by int32 Wq
+    void* W;                 // n x LDW, bit order: int32 W followed by packed W 65537 |
+                             // double followed by int32 Wq
     void* d;                 // 64 entries: int32 | double followed by int32 dq
-    void* delta;             // 2^M entries: int32 trie constants | FP64 Gray step constants
-                             // followed by the int32 trie constants of the quantized tables
+    void* delta;             // 2^M entries: int32 trie constants followed by the same times
+                             // 65537 | FP64 Gray step constants followed by the int32 trie
+                             // constants of the quantized tables
     RedDev* red;
     uint32_t* cand;          // chunks entries
     SlotDev* slots;          // chunks entries
@@ -55,11 +58,11 @@ struct InstDev {
 
 // Table regions of one problem (bytes), shared by the batch and serving layouts.
 __host__ __device__ inline size_t table_W_bytes(int kind, int n) {
-    return (size_t)n * LDW * (kind == KIND_I32 ? 4 : 12);
+    return (size_t)n * LDW * (kind == KIND_I32 ? 8 : 12);
 }
 __host__ __device__ inline size_t table_d_bytes(int kind) { return 64 * (kind == KIND_I32 ? 4 : 12); }
 __host__ __device__ inline size_t table_delta_bytes(int kind, int M) {
-    return (size_t)(1u << M) * (kind == KIND_I32 ? 4 : 12);
+    return (size_t)(1u << M) * (kind == KIND_I32 ? 8 : 12);
 }
 
 struct TileRange {
@@ -94,11 +97,14 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         int32_t* W = (int32_t*)I.W;
         int32_t* d = (int32_t*)I.d;
         int32_t* dl = (int32_t*)I.delta;
+        uint32_t* Wp = (uint32_t*)(W + (size_t)n * LDW);
+        uint32_t* dlp = (uint32_t*)(dl + (1 << M));
         long long acc = 0;
         for (int idx = tid; idx < n * LDW; idx += blockDim.x) {
             const int a = idx / LDW, b = idx - a * LDW;
             const long long w = w_elem<int32_t, long long>(Q, n, a, b);
             W[idx] = (int32_t)w;
+            Wp[idx] = pack_both((uint32_t)w);
             if (b > a && b < n) acc += w < 0 ? -w : w;
         }
         for (int a = tid; a < n; a += blockDim.x) {
@@ -106,8 +112,10 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             d[a] = (int32_t)v;
             acc += v < 0 ? -v : v;
         }
-        for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
+        for (uint32_t t = tid; t < (1u << M); t += blockDim.x) {
             dl[t] = (int32_t)trie_elem<int32_t, long long>(Q, n, M, t);
+            dlp[t] = pack_both((uint32_t)dl[t]);
+        }
         red_i[tid] = acc;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
@@ -256,7 +264,8 @@ static __device__ void finalize_problem(const InstDev& I, int lane) {
         I.result->rank = brank;
         I.result->state = rank_to_state(brank, I.B);
         I.result->status = I.status;
-        I.result->pad = (int32_t)cc;
+        I.result->path = I.kind == KIND_F64 ? QBX_PATH_FLOAT
+                         : (I.bound <= PAIR_BOUND && I.L > I.M) ? QBX_PATH_PAIR16 : QBX_PATH_INT32;
     }
 }
 
@@ -308,6 +317,22 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
     return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
 }
 
+// (E, lane rank, block key) minimum over the wave, made uniform
+__device__ __forceinline__ void wave_best_block(int32_t& Eb, uint64_t& lamb, uint32_t& Kb) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const int32_t E2 = __shfl_xor(Eb, off);
+        const uint64_t l2 = __shfl_xor(lamb, off);
+        const uint32_t K2 = __shfl_xor(Kb, off);
+        if (E2 < Eb || (E2 == Eb && (l2 < lamb || (l2 == lamb && K2 < Kb)))) { Eb = E2; lamb = l2; Kb = K2; }
+    }
+    Eb = (int32_t)uni32((uint32_t)Eb);
+    lamb = uni64(lamb);
+    Kb = uni32(Kb);
+}
+
+// Problems with energy bound <= PAIR_BOUND (and at least one outer bit) take the packed
+// pair walk, two states per 32-bit operation; the others the int32 walk.  Both give the same
+// lane keys; the winning block is re-walked by the int32 code either way.
 template <int M, int OMAX>
 static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
@@ -324,36 +349,48 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
         const int n = I.n, L = I.L, B = I.B, H = n - L;
         const int32_t* W = (const int32_t*)I.W;
         const int32_t* d = (const int32_t*)I.d;
-        // inner-block trie constants: uniform values kept in VGPRs (the lane-dependent zero
-        // offset keeps them out of the scalar register file)
-        int32_t dl[1 << M];
         const int32_t* dsrc = (const int32_t*)I.delta;
+        const uint32_t t0 = chunk * I.tpc;
+        const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
+        uint64_t rank = ~0ull;
+        if (I.status == 0 && I.bound <= PAIR_BOUND && L > M) {
+            const uint32_t* Wp = (const uint32_t*)(W + (size_t)n * LDW);
+            // packed trie constants: uniform values kept in VGPRs (the lane-dependent zero
+            // offset keeps them out of the scalar register file)
+            uint32_t dlp[1 << M];
+            const uint32_t* psrc = (const uint32_t*)(dsrc + (1 << M));
 #pragma unroll
-        for (int k = 0; k < (1 << M); ++k) dl[k] = dsrc[k + (lane & zero)];
-        if (I.status == 0) {
-            const uint32_t t0 = chunk * I.tpc;
-            const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
+            for (int k = 0; k < (1 << M); ++k) dlp[k] = psrc[k + (lane & zero)];
             for (uint32_t tile = t0; tile < t1; ++tile) {
                 const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
                 if (H >= 6 || lam < (1ull << H)) {
-                    const int64_t key = lane_best_block_i32<M, OMAX>(W, d, n, L, B, lam, dl);
+                    const int64_t key = lane_best_block_pair<M, OMAX>(W, Wp, d, n, L, B, lam, dlp);
                     if (key_E(key) < Eb) { Eb = key_E(key); lamb = lam; Kb = key_K(key); }
                 }
             }
+            wave_best_block(Eb, lamb, Kb);
+            // the wave's winning block: one int32 re-walk for its lowest inner rank, with the
+            // trie constants read from memory
+            if (Eb != 0x7fffffff)
+                rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dsrc);
+        } else {
+            int32_t dl[1 << M];
+#pragma unroll
+            for (int k = 0; k < (1 << M); ++k) dl[k] = dsrc[k + (lane & zero)];
+            if (I.status == 0) {
+                for (uint32_t tile = t0; tile < t1; ++tile) {
+                    const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
+                    if (H >= 6 || lam < (1ull << H)) {
+                        const int64_t key = lane_best_block_i32<M, OMAX>(W, d, n, L, B, lam, dl);
+                        if (key_E(key) < Eb) { Eb = key_E(key); lamb = lam; Kb = key_K(key); }
+                    }
+                }
+            }
+            wave_best_block(Eb, lamb, Kb);
+            // the wave's winning block is now uniform: re-walk it once for its lowest inner rank
+            if (Eb != 0x7fffffff)
+                rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dl);
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const int32_t E2 = __shfl_xor(Eb, off);
-            const uint64_t l2 = __shfl_xor(lamb, off);
-            const uint32_t K2 = __shfl_xor(Kb, off);
-            if (E2 < Eb || (E2 == Eb && (l2 < lamb || (l2 == lamb && K2 < Kb)))) { Eb = E2; lamb = l2; Kb = K2; }
-        }
-        // the wave's winning block is now uniform: re-walk it once for its lowest inner rank
-        Eb = (int32_t)uni32((uint32_t)Eb);
-        lamb = uni64(lamb);
-        Kb = uni32(Kb);
-        uint64_t rank = ~0ull;
-        if (Eb != 0x7fffffff)
-            rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dl);
         commit_tile(I, chunk, key_of_i64(Eb), (uint64_t)(int64_t)Eb, rank, lane);
     }
 }

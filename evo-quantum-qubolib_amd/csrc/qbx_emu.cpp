@@ -18,6 +18,7 @@ using namespace qbx;
 namespace qbx {
 unsigned long long emu_stat_flushes = 0, emu_stat_rewalks = 0;
 }
+static unsigned long long emu_stat_pair_lanes = 0;
 
 namespace {
 
@@ -30,6 +31,19 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
         for (int b = 0; b < LDW; ++b) W[(size_t)a * LDW + b] = w_elem<int32_t, int32_t>(Q, n, a, b);
     }
     for (uint32_t t = 0; t < (1u << M); ++t) dl[t] = trie_elem<int32_t, int32_t>(Q, n, M, t);
+    // the packed tables and the bound that selects the pair walk, as qbx_prep builds them
+    std::vector<uint32_t> Wp(W.size()), dlp(dl.size());
+    for (size_t i = 0; i < W.size(); ++i) Wp[i] = pack_both((uint32_t)W[i]);
+    for (size_t i = 0; i < dl.size(); ++i) dlp[i] = pack_both((uint32_t)dl[i]);
+    long long bound = 0;
+    for (int a = 0; a < n; ++a) {
+        bound += d[a] < 0 ? -(long long)d[a] : d[a];
+        for (int b = a + 1; b < n; ++b) {
+            const long long w = W[(size_t)a * LDW + b];
+            bound += w < 0 ? -w : w;
+        }
+    }
+    const bool pair = bound <= PAIR_BOUND && L > M;
     const int H = n - L;
     const uint64_t lanes = 1ull << H;
     // the device flow: every lane's best block key, the (E, lane rank, block key) winner
@@ -40,6 +54,13 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
     for (uint64_t lam = 0; lam < lanes; ++lam) {
         const int64_t key = lane_best_block_i32<M, OMAX>(W.data(), d.data(), n, L, B, lam, dl.data());
         if (key_E(key) < bE) { bE = key_E(key); bLam = lam; bK = key_K(key); }
+        // the packed pair walk must give every lane the same key
+        if (pair) {
+            const int64_t pk = lane_best_block_pair<M, OMAX>(W.data(), Wp.data(), d.data(), n, L, B, lam,
+                                                             dlp.data());
+            if (pk != key) return -5;
+            ++emu_stat_pair_lanes;
+        }
     }
     *E_out = bE;
     *rank_out = block_rank_hi(bLam, L, M, bK) |
@@ -106,7 +127,8 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
 extern "C" {
 
 // Returns 0 on success; -1 unsupported M; -2 unsupported L; -3 the two i32 flows disagree;
-// -4 float energies out of range.
+// -4 float energies out of range; -5 the packed pair walk's lane key differs from the
+// int32 walk's (checked for every lane when the problem's bound selects the pair walk).
 int qbx_emu_solve_i32(const int32_t* Q, int n, int B, int L, int M, int64_t* E, uint64_t* rank,
                       uint64_t* state) {
     int rc;
@@ -152,6 +174,13 @@ void qbx_emu_take_stats(uint64_t* flushes, uint64_t* rewalks) {
     *flushes = emu_stat_flushes;
     *rewalks = emu_stat_rewalks;
     emu_stat_flushes = emu_stat_rewalks = 0;
+}
+
+// lanes whose pair-walk key was checked since the last call
+uint64_t qbx_emu_take_pair_lanes(void) {
+    const uint64_t v = emu_stat_pair_lanes;
+    emu_stat_pair_lanes = 0;
+    return v;
 }
 
 uint64_t qbx_emu_state_to_rank(uint64_t s, int B) { return state_to_rank(s, B); }

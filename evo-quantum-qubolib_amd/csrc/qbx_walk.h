@@ -552,6 +552,213 @@ QBX_HD LaneOutI32 walk_lane_i32(const int32_t* __restrict__ W, const int32_t* __
     return out;
 }
 
+// ------------------------------------------------ packed pair walk (energies in 16 bits)
+// For integer problems whose energy bound (sum |d| + sum_{a<b} |W|) is at most PAIR_BOUND,
+// every energy fits in 16 bits and a lane walks two blocks at once: the states whose pair
+// bit (state bit M, the lowest outer bit) is 0 in the low half of a 32-bit word, those with
+// it set in the high half.  A packed value is V = x0 + x1 2^16 (mod 2^32); the map is
+// linear, so the int32 walk carries over unchanged onto packed values (fields
+// f0 + f1 2^16, a coupling shared by both halves as w 65537) and one v_add3_u32 now
+// advances two states.  Only energies of actual states are ever unpacked, and those fit:
+// they are stored biased, E' = E + PAIR_BIAS in [0x0401, 0x7bff].  Block minima are taken
+// per half by v_pk_minimum3_f16: those 16-bit patterns are positive normal halfs, whose
+// order is their integer order, and the minimum of normal operands is one of them,
+// unchanged.  The packed walk covers outer bits 1 .. O-1 in Gray order; the pair bit never
+// flips, so its couplings are folded into the fields of the walked bits once, at set-up.
+// Block minima, keys and the winning block's re-walk are those of the int32 walk, so the
+// result is the int32 walk's (E, rank) minimum exactly.
+constexpr int32_t PAIR_BIAS = 0x4000;
+constexpr int64_t PAIR_BOUND = 0x4000 - 0x0401;   // 15359 (QBX_PAIR_BOUND)
+
+QBX_HD uint32_t pack_both(uint32_t w) { return w + (w << 16); }
+
+// per-half minimum of three packed values (every half a biased energy)
+QBX_HD uint32_t pmin3(uint32_t m, uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 r = __builtin_elementwise_minimum(
+        __builtin_elementwise_minimum(__builtin_bit_cast(h2, m), __builtin_bit_cast(h2, a)),
+        __builtin_bit_cast(h2, b));
+    return __builtin_bit_cast(uint32_t, r);
+#else
+    uint32_t lo = m & 0xffffu, hi = m >> 16;
+    lo = (a & 0xffffu) < lo ? (a & 0xffffu) : lo;
+    lo = (b & 0xffffu) < lo ? (b & 0xffffu) : lo;
+    hi = (a >> 16) < hi ? (a >> 16) : hi;
+    hi = (b >> 16) < hi ? (b >> 16) : hi;
+    return lo | (hi << 16);
+#endif
+}
+
+template <int M, int OMAX>
+struct LanePair {
+    uint32_t E0;         // packed energies of the two block bases
+    uint32_t f[M];       // packed fields of the inner bits
+    uint32_t u[OMAX];    // packed upper fields of outer bit q + 1 (the pair bit included)
+};
+
+template <int M, unsigned P>
+QBX_HD void ptrie_even(uint32_t EP, const uint32_t* f, const uint32_t* cc, uint32_t& mn);
+
+template <int M, unsigned P, int K>
+QBX_HD void ptrie_children(uint32_t EP, const uint32_t* f, const uint32_t* cc, uint32_t& mn) {
+    if constexpr (K < M && K < ctz_c(P)) {
+        constexpr unsigned C = P | (1u << K);
+        ptrie_even<M, C>(EP + f[K] + cc[C], f, cc, mn);
+        ptrie_children<M, P, K + 1>(EP, f, cc, mn);
+    }
+}
+
+template <int M, unsigned P>
+QBX_HD void ptrie_even(uint32_t EP, const uint32_t* f, const uint32_t* cc, uint32_t& mn) {
+    const uint32_t Eodd = EP + f[0] + cc[P | 1u];
+    mn = pmin3(mn, EP, Eodd);
+    ptrie_children<M, P, 1>(EP, f, cc, mn);
+}
+
+template <int M>
+QBX_HD uint32_t block_min_pair(uint32_t E0, const uint32_t* f, const uint32_t* cc) {
+    uint32_t mn = E0;
+    ptrie_even<M, 0u>(E0, f, cc, mn);
+    return mn;
+}
+
+// Row M+1+I of the packed couplings: the inner entries and the walked outer bits below I.
+template <int M>
+struct WRowPair {
+    uint32_t w[M + 4];
+};
+
+template <int M, int I>
+QBX_HD void load_wrow_pair(WRowPair<M>& r, const uint32_t* __restrict__ Wp) {
+    const uint32_t* WJ = Wp + (M + 1 + I) * LDW;
+#pragma unroll
+    for (int k = 0; k < M; ++k) r.w[k] = WJ[k];
+#pragma unroll
+    for (int q = 0; q < I; ++q) r.w[M + 1 + q] = WJ[M + 1 + q];
+}
+
+template <int M, int OMAX, int I>
+QBX_HD void trans_row_pair(LanePair<M, OMAX>& s, const WRowPair<M>& r, bool up) {
+    constexpr int J = M + 1 + I;
+    const uint32_t field = s.u[I] + (I > 0 ? r.w[J - 1] : 0u);
+    if (up) {
+        s.E0 += field;
+#pragma unroll
+        for (int k = 0; k < M; ++k) s.f[k] += r.w[k];
+#pragma unroll
+        for (int q = 0; q < I; ++q) s.u[q] += r.w[M + 1 + q];
+    } else {
+        s.E0 -= field;
+#pragma unroll
+        for (int k = 0; k < M; ++k) s.f[k] -= r.w[k];
+#pragma unroll
+        for (int q = 0; q < I; ++q) s.u[q] -= r.w[M + 1 + q];
+    }
+}
+
+// run-time walked bit i >= GROUP_BITS (state bit M + 1 + i)
+template <int M, int OMAX>
+QBX_HD void trans_dyn_pair(LanePair<M, OMAX>& s, const uint32_t* __restrict__ WJ, int i, bool up) {
+    const int J = M + 1 + i;
+    uint32_t uJ = 0;
+#pragma unroll
+    for (int q = 0; q < OMAX; ++q) if (q == i) uJ = s.u[q];
+    const uint32_t sg = up ? 1u : 0xffffffffu;
+    s.E0 += sg * (uJ + WJ[J - 1]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] += sg * WJ[k];
+#pragma unroll
+    for (int q = 0; q < OMAX; ++q) if (q < i) s.u[q] += sg * WJ[M + 1 + q];
+}
+
+// Packed block c holds the outer configurations (gray(c) << 1) | h, h = the pair bit; in
+// the int32 walk's numbering that is block (c << 1) | (h ^ (c & 1)).
+template <int M, int OMAX, int J>
+QBX_HD bool group_step_pair(LanePair<M, OMAX>& s, const uint32_t* __restrict__ Wp, const uint32_t* dl,
+                            uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best,
+                            WRowPair<M>& row) {
+    const uint32_t c = c0 + (uint32_t)J;
+    if (J > 0) {
+        if (c >= nblk) return false;
+        constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
+        const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
+                                             : (((c0 >> GROUP_BITS) & 1u) == 0u);
+        trans_row_pair<M, OMAX, i>(s, row, up);
+    }
+    if (J + 1 < (1 << GROUP_BITS)) load_wrow_pair<M, __builtin_ctz((unsigned)(J + 1))>(row, Wp);
+    const uint32_t mp = block_min_pair<M>(s.E0, s.f, dl);
+    const uint32_t lo = mp & 0xffffu, hi = mp >> 16;
+    const uint32_t m = lo < hi ? lo : hi;
+    if (__builtin_expect((int32_t)m <= best.E, 0)) {
+        const uint32_t c0f = (c << 1) | (c & 1u);
+        if (lo == m) {
+            const uint32_t K = block_key(c0f, SBk, pmaskS);
+            if ((int32_t)m < best.E || K < best.K) { best.E = (int32_t)m; best.K = K; }
+        }
+        if (hi == m) {
+            const uint32_t K = block_key(c0f ^ 1u, SBk, pmaskS);
+            if ((int32_t)m < best.E || K < best.K) { best.E = (int32_t)m; best.K = K; }
+        }
+    }
+    return true;
+}
+
+template <int M, int OMAX, int... Js>
+QBX_HD void group_pair(LanePair<M, OMAX>& s, const uint32_t* __restrict__ Wp, const uint32_t* dl,
+                       uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best,
+                       std::integer_sequence<int, Js...>) {
+    WRowPair<M> row;
+    (void)(group_step_pair<M, OMAX, Js>(s, Wp, dl, c0, nblk, SBk, pmaskS, best, row) && ...);
+}
+
+// Set-up: the int32 walk's first block (pair bit 0) in the low halves; the high halves are
+// the same states with the pair bit set.
+template <int M, int OMAX>
+QBX_HD void lane_setup_pair(LanePair<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* __restrict__ d,
+                            int n, int L, int B, uint64_t lam) {
+    LaneI32<M, OMAX> a;
+    lane_setup_i32<M, OMAX>(a, W, d, n, L, B, lam);
+    const int32_t* WM = W + M * LDW;
+    const uint32_t e0 = (uint32_t)a.E0 + (uint32_t)PAIR_BIAS;
+    const uint32_t e1 = e0 + (uint32_t)a.u[0];
+    s.E0 = e0 + (e1 << 16);
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] = (uint32_t)a.f[k] + (((uint32_t)a.f[k] + (uint32_t)WM[k]) << 16);
+#pragma unroll
+    for (int q = 0; q + 1 < OMAX; ++q)
+        s.u[q] = (uint32_t)a.u[q + 1] + (((uint32_t)a.u[q + 1] + (uint32_t)WM[M + 1 + q]) << 16);
+    s.u[OMAX - 1] = 0u;
+}
+
+// The int32 walk's lane key (lane_best_block_i32), from the packed walk.  Needs L > M and
+// bound <= PAIR_BOUND; Wp, dlp: the couplings and trie constants times 65537.
+template <int M, int OMAX>
+QBX_HD int64_t lane_best_block_pair(const int32_t* __restrict__ W, const uint32_t* __restrict__ Wp,
+                                    const int32_t* __restrict__ d, int n, int L, int B, uint64_t lam,
+                                    const uint32_t* dlp) {
+    static_assert(OMAX > GROUP_BITS, "OMAX must cover the unrolled transition bits");
+    const int O = L - M;
+    const uint64_t p = lane_prefix(lam, L, B);
+    LanePair<M, OMAX> s;
+    lane_setup_pair<M, OMAX>(s, W, d, n, L, B, lam);
+    const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
+    const uint32_t pmaskS = (parity64(p) && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
+    LaneBest best{0x7fffffff, 0xffffffffu};
+    const uint32_t nblk = 1u << (O - 1);
+    for (uint32_t c0 = 0;;) {
+        group_pair<M, OMAX>(s, Wp, dlp, c0, nblk, SBk, pmaskS, best,
+                            std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
+        c0 += 1u << GROUP_BITS;
+        if (c0 >= nblk) break;
+        const int i = ctz32(c0);
+        const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
+        trans_dyn_pair<M, OMAX>(s, Wp + (M + 1 + i) * LDW, i, up);
+    }
+    const int32_t E = best.E - PAIR_BIAS;
+    return (int64_t)(((uint64_t)(uint32_t)E << 32) | (uint64_t)best.K);
+}
+
 // ----------------------------------------------- float problems: int32 filter, FP64 walk
 // The block filter runs on quantized tables Wq = rint(W 2^s), dq = rint(d 2^s) in exact
 // int32 arithmetic (the int walk above, unchanged).  Each of the terms d_a, W_ab that a

@@ -15,7 +15,7 @@ import numpy as np
 __all__ = [
     "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
     "ServerStats", "Pcg64State", "GEN_DTYPES", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
-    "tuning_path",
+    "tuning_path", "PATHS", "PAIR_BOUND",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libqbx.so"
@@ -49,6 +49,11 @@ class Opts(ctypes.Structure):
     ]
 
 
+# qbx_result_t.path: the walk that ran (include/qbx.h QBX_PATH_*)
+PATHS = {0: "int32", 1: "pair16", 2: "float"}
+PAIR_BOUND = 15359       # QBX_PAIR_BOUND: integer bound up to which the packed pair walk runs
+
+
 class Result(ctypes.Structure):
     _fields_ = [
         ("E_int", ctypes.c_int64),
@@ -64,6 +69,8 @@ class Result(ctypes.Structure):
         ("inner_bits", ctypes.c_int32),
         ("lane_bits", ctypes.c_int32),
         ("status", ctypes.c_int32),
+        ("path", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

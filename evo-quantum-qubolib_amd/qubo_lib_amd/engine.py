@@ -105,7 +105,8 @@ def _finish(p: Prepared, r: _native.Result) -> RawResult:
     else:
         E = float(r.E)
     info = dict(kernel_ms=r.kernel_ms, states=int(r.states), tiles=int(r.tiles), walk_bits=r.walk_bits,
-                inner_bits=r.inner_bits, lane_bits=r.lane_bits, path=p.kind, bound=p.bound)
+                inner_bits=r.inner_bits, lane_bits=r.lane_bits, path=p.kind, bound=p.bound,
+                walk=_native.PATHS.get(r.path))
     return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=p.n, tie_B=r.tie_B, kind=p.kind, info=info)
 
 
@@ -157,7 +158,8 @@ def raw_from_native(r: _native.Result, integer: bool) -> RawResult:
     kind = "i32" if integer else "f64"
     E = int(r.E_int) if integer else float(r.E)
     info = dict(kernel_ms=r.kernel_ms, states=int(r.states), tiles=int(r.tiles), walk_bits=r.walk_bits,
-                inner_bits=r.inner_bits, lane_bits=r.lane_bits, path=kind, bound=None)
+                inner_bits=r.inner_bits, lane_bits=r.lane_bits, path=kind, bound=None,
+                walk=_native.PATHS.get(r.path))
     return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=int(r.n), tie_B=r.tie_B, kind=kind, info=info)
 
 

@@ -60,7 +60,18 @@ typedef struct {
     int32_t inner_bits;
     int32_t lane_bits;
     int32_t status;       /* 0 = ok, QBX_ERANGE = energy bound outside the int32 path */
+    int32_t path;         /* walk that ran: QBX_PATH_INT32, QBX_PATH_PAIR16 or QBX_PATH_FLOAT */
+    int32_t reserved;
 } qbx_result_t;
+
+/* qbx_result_t.path.  Integer problems whose energy bound (sum |Q_ii| + sum_{i<j} |Q_ij + Q_ji|)
+ * is at most QBX_PAIR_BOUND walk two states per 32-bit operation (16-bit energies, same
+ * answers); other integer problems walk in int32; float problems take the quantized filter
+ * with FP64 re-walks. */
+#define QBX_PATH_INT32 0
+#define QBX_PATH_PAIR16 1
+#define QBX_PATH_FLOAT 2
+#define QBX_PAIR_BOUND 15359
 
 typedef struct {
     char name[64];        /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */

@@ -57,3 +57,36 @@ def random_instance(rng, n, kind):
 
 
 KINDS = ("uniform", "maxcut", "ternary", "sparse", "zero", "positive")
+
+
+PAIR_BOUND = 15359
+
+
+def energy_bound(Q):
+    Q = np.asarray(Q, dtype=np.int64)
+    up = np.triu(Q, 1) + np.triu(Q.T, 1)
+    return int(np.abs(np.diag(Q)).sum() + np.abs(up).sum())
+
+
+def pair_case(rng, n, trial):
+    """Integer problems whose energies fit the packed pair walk, including both ends of its
+    range (an all-ones state reaching -PAIR_BOUND or +PAIR_BOUND)."""
+    kind = trial % 5
+    if kind == 0:
+        return random_instance(rng, n, "maxcut")
+    if kind == 1:
+        return random_instance(rng, n, "ternary")
+    if kind == 2:      # every entry <= 0: the all-ones state sits at -bound
+        Q = -rng.integers(0, 60, (n, n))
+    elif kind == 3:    # every entry >= 0: the all-ones state sits at +bound
+        Q = rng.integers(0, 60, (n, n))
+    else:
+        Q = rng.integers(-90, 91, (n, n))
+    Q = np.asarray(Q, dtype=np.int64)
+    b = energy_bound(Q)
+    if b > PAIR_BOUND // 2:     # scale into range
+        Q = Q * (PAIR_BOUND // 2) // b
+    # grow |Q[0, 0]| until the bound is exactly PAIR_BOUND
+    gap = PAIR_BOUND - energy_bound(Q)
+    Q[0, 0] += -gap if Q[0, 0] < 0 or kind == 2 else gap
+    return Q

@@ -1,0 +1,91 @@
+"""The packed pair walk on the GPU: integer problems with energy bound <= 15359 walk two
+states per 32-bit operation (16-bit biased energies, per-half minima by
+v_pk_minimum3_f16).  Answers must be the oracle's bit for bit, at both ends of the 16-bit
+range, for every kernel variant and walk width, and equal to the int32 walk's on the same
+problem scaled out of the pair range.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import PAIR_BOUND, energy_bound, pair_case, random_instance
+from qubo_lib_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def flush(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("inner", [7, 6, 4, 1])
+def test_pair_walk_vs_oracle(gpu_ctx, inner):
+    rng = np.random.default_rng(900 + inner)
+    omax = {7: 13, 6: 14, 4: 16, 1: 19}[inner]
+    for trial in range(15):
+        n = int(rng.integers(max(inner + 1, 10), 23))
+        Q = pair_case(rng, n, trial)
+        assert energy_bound(Q) <= PAIR_BOUND
+        B = [None, 0, int(rng.integers(0, n + 1))][trial % 3]
+        ref = oracle.brute_i64(Q, B)
+        for L in sorted({inner + 1, min(n, inner + 5), min(n, inner + omax)}):
+            flush("pair", inner, trial, n, B, L)
+            got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=inner)
+            assert (got.E, got.state) == ref, (inner, trial, n, B, L)
+            assert got.info["walk"] == "pair16"
+
+
+def test_pair_range_edges(gpu_ctx):
+    """Energies at exactly -15359 and +15359, the bound just above the pair range (int32
+    walk), and no outer bit to pair on (int32 walk)."""
+    n = 20
+    rng = np.random.default_rng(5)
+    low = -rng.integers(0, 40, (n, n))
+    low[0, 0] -= PAIR_BOUND - energy_bound(low)
+    high = rng.integers(0, 40, (n, n))
+    high[0, 0] += PAIR_BOUND - energy_bound(high)
+    for Q, walk in ((low, "pair16"), (high, "pair16"), (-high, "pair16")):
+        assert energy_bound(Q) == PAIR_BOUND
+        got = engine.solve_matrix(Q)
+        assert (got.E, got.state) == oracle.brute_i64(Q), walk
+        assert got.info["walk"] == walk
+    over = low.copy()
+    over[0, 0] -= 1
+    got = engine.solve_matrix(over)
+    assert got.info["walk"] == "int32" and (got.E, got.state) == oracle.brute_i64(over)
+    Q = random_instance(rng, 12, "ternary")
+    got = engine.solve_matrix(Q, walk_bits=7, inner_bits=7)
+    assert got.info["walk"] == "int32" and (got.E, got.state) == oracle.brute_i64(Q)
+
+
+def test_pair_matches_int32_walk_large(gpu_ctx):
+    """n = 30 .. 34 MaxCut and ternary problems: the pair walk's answer equals the int32
+    walk's on the same problem scaled by k (same argmin, energies x k, bound out of range)."""
+    rng = np.random.default_rng(6)
+    for n, kind in ((30, "maxcut"), (32, "ternary"), (34, "maxcut")):
+        Q = random_instance(rng, n, kind)
+        k = PAIR_BOUND // energy_bound(Q) + 1
+        for B in (None, 0):
+            flush("pair large", n, kind, B, k)
+            a = engine.solve_matrix(Q, tie_B=B)
+            b = engine.solve_matrix(Q * k, tie_B=B)
+            assert a.info["walk"] == "pair16" and b.info["walk"] == "int32"
+            assert (a.E * k, a.state) == (b.E, b.state), (n, kind, B)
+
+
+def test_pair_in_batches_and_shares(gpu_ctx):
+    """Batches mixing pair and int32 problems, and the shares of a pair solve."""
+    rng = np.random.default_rng(7)
+    Qs = []
+    for i in range(40):
+        n = int(rng.integers(8, 21))
+        Qs.append(pair_case(rng, n, i) if i % 2 else random_instance(rng, n, "uniform") * 200)
+    got = engine.solve_matrices(Qs)
+    for i, (Q, r) in enumerate(zip(Qs, got)):
+        assert (r.E, r.state) == oracle.brute_i64(Q), i
+        pair = energy_bound(Q) <= PAIR_BOUND and r.info["walk_bits"] > r.info["inner_bits"]
+        assert r.info["walk"] == ("pair16" if pair else "int32"), i
+    Q = random_instance(rng, 28, "maxcut")
+    full = engine.solve_matrix(Q, walk_bits=16)
+    merged = engine.merge_parts([engine.solve_part(Q, k, 8, walk_bits=16) for k in range(8)])
+    assert (merged.E, merged.state) == (full.E, full.state)

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     # sizes of the ABI structs as laid out by the C compiler for include/qbx.h
     assert ctypes.sizeof(_native.Opts) == 16
-    assert ctypes.sizeof(_native.Result) == 8 * 7 + 4 * 6 + 0  # 80 bytes
+    assert ctypes.sizeof(_native.Result) == 8 * 7 + 4 * 8  # 88 bytes
     assert ctypes.sizeof(_native.DevInfo) == 64 + 4 * 4 + 8 + 8
 
 

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import KINDS, itertools_min, random_instance
+from helpers import KINDS, PAIR_BOUND, energy_bound, itertools_min, pair_case, random_instance
 
 EMU = Path(__file__).resolve().parent / "_build" / "libqbx_emu.so"
 P = ctypes.POINTER
@@ -138,3 +138,33 @@ def test_float_filter_walk(emu, M):
     f, r = take_stats(emu)
     assert f > 0 and r > 0
     assert rewalks > 0
+
+
+
+@pytest.mark.parametrize("M", [1, 4, 6, 7])
+def test_pair_walk_matches_int32_walk(emu, M):
+    """Problems with bound <= 15359 take the packed pair walk (two states per 32-bit word).
+    The emulator runs it beside the int32 walk for every lane and fails on any differing
+    lane key; the answers must also be the brute force's."""
+    rng = np.random.default_rng(500 + M)
+    emu.qbx_emu_take_pair_lanes.restype = ctypes.c_uint64
+    emu.qbx_emu_take_pair_lanes()
+    at_bound = 0
+    for trial in range(30):
+        n = int(rng.integers(M + 1, 15))
+        Q = pair_case(rng, n, trial)
+        assert energy_bound(Q) <= PAIR_BOUND
+        at_bound += energy_bound(Q) == PAIR_BOUND
+        B = [min(n, 20), 0, int(rng.integers(0, n + 1))][trial % 3]
+        ref = oracle.brute_i64(Q, B)
+        for L in sorted({M + 1, min(n, M + 5), min(n, M + 6), n}):
+            assert run_i32(emu, Q, B, L, M) == ref, (n, B, L, M, trial)
+            assert emu.qbx_emu_take_pair_lanes() == 2 ** (n - L), (n, L, M)
+    assert at_bound >= 10
+    # above the bound, or with no outer bit to pair on, the int32 walk runs alone
+    Q = np.full((10, 10), 200)
+    assert energy_bound(Q) > PAIR_BOUND
+    assert run_i32(emu, Q, 10, M + 2, M) == oracle.brute_i64(Q, 10)
+    Q = random_instance(rng, 9, "ternary")
+    assert run_i32(emu, Q, 9, M, M) == oracle.brute_i64(Q, 9)
+    assert emu.qbx_emu_take_pair_lanes() == 0
