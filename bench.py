@@ -122,12 +122,18 @@ def measured_traffic(workload: str):
 
 def exhaustive(args, ctx, devinfo):
     from qubo_lib_amd import engine
-    from qubo_lib_amd.problems.generators import symmetric_int_qubo, symmetric_normal_qubo
+    from qubo_lib_amd.problems.generators import maxcut_gnp, symmetric_int_qubo, symmetric_normal_qubo
     if args.workload == "float":
         n = 34
         Q = symmetric_normal_qubo(n, args.seed)
         workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: quantized int32 filter walk, FP64 exact re-walk"
         dtype = "f32"
+    elif args.workload == "maxcut":
+        n = 40
+        Q = maxcut_gnp(n, 0.5, seed=args.seed)
+        workload = (f"MaxCut G(n=40, p=0.5) unit weights (seed {args.seed}), 2^40 states, exact: energy bound "
+                    f"<= 15359, packed pair walk (two 16-bit states per 32-bit op)")
+        dtype = "int16x2"
     else:
         n = 40
         Q = symmetric_int_qubo(n, args.seed, -127, 127, np.int16)
@@ -159,9 +165,20 @@ def exhaustive(args, ctx, devinfo):
     assert len(set(results)) == 1, "answers differ between steps"
     E, state = results[0]
     parity = {"E": E, "state": state}
+    if args.workload == "maxcut":
+        # the pair walk does two 16-bit operations per packed instruction: its peak is twice
+        # the int32 one; parity against the int32 walk on the same problem scaled out of the
+        # pair range (same argmin, energies times k)
+        peak *= 2
+        from qubo_lib_amd import _native
+        k = _native.PAIR_BOUND // max(1, int(prep.bound)) + 1
+        ref = engine.solve_matrix(Q * k, **kw)
+        parity.update(walk=r.info["walk"], int32_walk={"E": ref.E, "state": ref.state, "scale": k,
+                                                       "walk": ref.info["walk"], "kernel_ms": ref.info["kernel_ms"]},
+                      matches_int32_walk=bool(ref.E == E * k and ref.state == state))
     gold = ROOT / "tests" / "golden" / "oracle_goldens.json"
-    key = f"sym_int16_n40_s{args.seed}" if args.workload != "float" else f"sym_normal_n34_s{args.seed}"
-    if gold.exists():
+    key = {"float": f"sym_normal_n34_s{args.seed}", "maxcut": None}.get(args.workload, f"sym_int16_n40_s{args.seed}")
+    if gold.exists() and key:
         g = json.loads(gold.read_text()).get(key)
         if g is not None:
             parity["oracle_state"] = g["state"]
@@ -170,7 +187,9 @@ def exhaustive(args, ctx, devinfo):
             parity["matches_oracle"] = bool(g["state"] == state and
                                             (g["E"] == E if isinstance(E, int) else abs(g["E"] - E) < 1e-9 * max(1, abs(E))))
     line = {
-        "metric": "states/s at n=40 int16 (exact)" if args.workload != "float" else "states/s at n=34 float32 (FP64-exact argmin)",
+        "metric": {"float": "states/s at n=34 float32 (FP64-exact argmin)",
+                   "maxcut": "states/s at n=40 MaxCut G(40, 0.5) (exact, packed pair walk)"}.get(
+                       args.workload, "states/s at n=40 int16 (exact)"),
         "value": value,
         "unit": "states/s",
         "n_gpus": 1,
@@ -189,9 +208,12 @@ def exhaustive(args, ctx, devinfo):
                    "device": devinfo["name"], "compute_units": devinfo["compute_units"],
                    "clock_mhz": devinfo["clock_khz"] / 1000.0},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic("exhaustive" if args.workload != "float" else "float"),
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic({"float": "float", "maxcut": "maxcut"}.get(args.workload, "exhaustive")),
                      "ops_per_state": OPS_PER_STATE,
-                     "note": "kernel time = HIP events around prep+walk on the library stream"},
+                     "note": "kernel time = HIP events around prep+walk on the library stream" +
+                             ("; peak = packed 16-bit VALU ops (2 per lane per instruction)"
+                              if args.workload == "maxcut" else "")},
         "parity": parity,
         "host_inclusive": {"value": value_host, "unit": "states/s",
                            "note": "wall clock around the K solve calls: staging Q from host memory, "
@@ -438,7 +460,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["exhaustive", "float", "batch", "serve", "copy"], default="exhaustive")
+    ap.add_argument("--workload", choices=["exhaustive", "float", "maxcut", "batch", "serve", "copy"],
+                    default="exhaustive")
     ap.add_argument("--serve-count", type=int, default=20000)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[2000.0, 20000.0, 100000.0])
     ap.add_argument("--copy-mib", type=int, default=256)
