@@ -133,7 +133,7 @@ def exhaustive(args, ctx, devinfo):
         Q = maxcut_gnp(n, 0.5, seed=args.seed)
         workload = (f"MaxCut G(n=40, p=0.5) unit weights (seed {args.seed}), 2^40 states, exact: energy bound "
                     f"<= 15359, packed pair walk (two 16-bit states per 32-bit op)")
-        dtype = "int16x2"
+        dtype = "int32"
     else:
         n = 40
         Q = symmetric_int_qubo(n, args.seed, -127, 127, np.int16)
@@ -161,17 +161,19 @@ def exhaustive(args, ctx, devinfo):
     value = states * args.steps / (float(np.sum(kms)) * 1e-3)
     value_host = states * args.steps / wall
     peak = devinfo["valu_peak_ops"]
+    # the packed walks do two 16-bit operations per lane and instruction: their ceiling is
+    # the packed 16-bit VALU rate, twice the int32 one
+    packed = r.info["walk"] in ("pair16", "pair16_rebased")
+    if packed:
+        peak *= 2
     achieved = states * OPS_PER_STATE / (kernel_ms * 1e-3)
     assert len(set(results)) == 1, "answers differ between steps"
     E, state = results[0]
     parity = {"E": E, "state": state}
     if args.workload == "maxcut":
-        # the pair walk does two 16-bit operations per packed instruction: its peak is twice
-        # the int32 one; parity against the int32 walk on the same problem scaled out of the
-        # pair range (same argmin, energies times k)
-        peak *= 2
-        from qubo_lib_amd import _native
-        k = _native.PAIR_BOUND // max(1, int(prep.bound)) + 1
+        # parity against the int32 walk on the same problem scaled out of the packed walks'
+        # range (same argmin, energies times k)
+        k = 4096   # couplings of 8192 per edge: too large for either packed walk
         ref = engine.solve_matrix(Q * k, **kw)
         parity.update(walk=r.info["walk"], int32_walk={"E": ref.E, "state": ref.state, "scale": k,
                                                        "walk": ref.info["walk"], "kernel_ms": ref.info["kernel_ms"]},
@@ -199,11 +201,12 @@ def exhaustive(args, ctx, devinfo):
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": dtype,
+        "dtype": "int16x2" if packed else dtype,
         "data": "synthetic",
         "config": {"workload": workload, "n": n, "seed": args.seed, "states_per_step": int(states),
                    "walk_bits": r.info["walk_bits"], "inner_bits": r.info["inner_bits"],
                    "lane_bits": r.info["lane_bits"], "tiles": r.info["tiles"], "tie_B": r.tie_B,
+                   "walk": r.info["walk"],
                    "kernel_ms_mean": kernel_ms, "tuning": str(ctx.tuning_file.name) if ctx.tuning_file else "built-in rule",
                    "device": devinfo["name"], "compute_units": devinfo["compute_units"],
                    "clock_mhz": devinfo["clock_khz"] / 1000.0},
@@ -212,8 +215,8 @@ def exhaustive(args, ctx, devinfo):
                      "traffic": measured_traffic({"float": "float", "maxcut": "maxcut"}.get(args.workload, "exhaustive")),
                      "ops_per_state": OPS_PER_STATE,
                      "note": "kernel time = HIP events around prep+walk on the library stream" +
-                             ("; peak = packed 16-bit VALU ops (2 per lane per instruction)"
-                              if args.workload == "maxcut" else "")},
+                             (f"; walk {r.info['walk']}: peak = packed 16-bit VALU ops (2 per lane per instruction)"
+                              if packed else "")},
         "parity": parity,
         "host_inclusive": {"value": value_host, "unit": "states/s",
                            "note": "wall clock around the K solve calls: staging Q from host memory, "

@@ -54,6 +54,8 @@ struct InstDev {
     double qmargin;          // float kind: filter margin in quantized units
     int64_t bound;           // sum |d| + sum_{a<b} |W|  (int kind; quantized for float)
     double fbound;           // same in double (float kind)
+    int32_t path;            // int kind, set by qbx_prep: QBX_PATH_INT32 | _PAIR16 | _PAIR16_REBASED
+    int32_t slack;           // rebased pair walk: PAIR_BOUND - D, the guard's room
 };
 
 // Table regions of one problem (bytes), shared by the batch and serving layouts.
@@ -125,6 +127,18 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         if (tid == 0) {
             I.bound = red_i[0];
             I.status = red_i[0] > 0x7ffffffeLL ? QBX_ERANGE : 0;
+            // the walk: packed pairs of absolute 16-bit energies, of block-relative ones, or int32
+            I.path = QBX_PATH_INT32;
+            I.slack = 0;
+            if (I.L > M && red_i[0] <= PAIR_BOUND) {
+                I.path = QBX_PATH_PAIR16;
+            } else if (I.status == 0 && I.L > M) {
+                const int64_t D = rp_guard_D(W, M, I.L);
+                if (rp_worth_it(W, d, n, M, I.L, D)) {
+                    I.path = QBX_PATH_PAIR16_REBASED;
+                    I.slack = (int32_t)(PAIR_BOUND - D);
+                }
+            }
         }
     } else {
         // FP64 tables for the exact re-walks, then their quantized int32 copies for the filter
@@ -264,8 +278,7 @@ static __device__ void finalize_problem(const InstDev& I, int lane) {
         I.result->rank = brank;
         I.result->state = rank_to_state(brank, I.B);
         I.result->status = I.status;
-        I.result->path = I.kind == KIND_F64 ? QBX_PATH_FLOAT
-                         : (I.bound <= PAIR_BOUND && I.L > I.M) ? QBX_PATH_PAIR16 : QBX_PATH_INT32;
+        I.result->path = I.kind == KIND_F64 ? QBX_PATH_FLOAT : I.path;
     }
 }
 
@@ -330,9 +343,9 @@ __device__ __forceinline__ void wave_best_block(int32_t& Eb, uint64_t& lamb, uin
     Kb = uni32(Kb);
 }
 
-// Problems with energy bound <= PAIR_BOUND (and at least one outer bit) take the packed
-// pair walk, two states per 32-bit operation; the others the int32 walk.  Both give the same
-// lane keys; the winning block is re-walked by the int32 code either way.
+// The walk qbx_prep chose per problem: the packed pair walk (bound <= PAIR_BOUND), the
+// rebased pair walk (block-relative 16-bit energies, guarded per group), or the int32 walk.
+// All give the same lane keys; the winning block is re-walked by the int32 code either way.
 template <int M, int OMAX>
 static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
@@ -353,7 +366,8 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
         const uint32_t t0 = chunk * I.tpc;
         const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
         uint64_t rank = ~0ull;
-        if (I.status == 0 && I.bound <= PAIR_BOUND && L > M) {
+        const int path = I.path;
+        if (I.status == 0 && (path == QBX_PATH_PAIR16 || path == QBX_PATH_PAIR16_REBASED)) {
             const uint32_t* Wp = (const uint32_t*)(W + (size_t)n * LDW);
             // packed trie constants: uniform values kept in VGPRs (the lane-dependent zero
             // offset keeps them out of the scalar register file)
@@ -364,7 +378,10 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
             for (uint32_t tile = t0; tile < t1; ++tile) {
                 const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
                 if (H >= 6 || lam < (1ull << H)) {
-                    const int64_t key = lane_best_block_pair<M, OMAX>(W, Wp, d, n, L, B, lam, dlp);
+                    const int64_t key = path == QBX_PATH_PAIR16
+                                            ? lane_best_block_pair<M, OMAX>(W, Wp, d, n, L, B, lam, dlp)
+                                            : lane_best_block_rp<M, OMAX>(W, Wp, d, n, L, B, lam, dlp, I.slack,
+                                                                          nullptr);
                     if (key_E(key) < Eb) { Eb = key_E(key); lamb = lam; Kb = key_K(key); }
                 }
             }

@@ -19,6 +19,8 @@ namespace qbx {
 unsigned long long emu_stat_flushes = 0, emu_stat_rewalks = 0;
 }
 static unsigned long long emu_stat_pair_lanes = 0;
+static unsigned long long emu_stat_rp_lanes = 0, emu_stat_rp_int32_groups = 0;
+static int emu_rp_slack_override = -1;
 
 namespace {
 
@@ -44,6 +46,12 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
         }
     }
     const bool pair = bound <= PAIR_BOUND && L > M;
+    // the rebased pair walk runs wherever its guard constant leaves room (the device also
+    // asks rp_worth_it; here every eligible problem is checked), with the slack overridable
+    // so that tests can force the int32 groups
+    const int64_t D = L > M ? rp_guard_D(W.data(), M, L) : PAIR_BOUND + 1;
+    const bool rp = L > M && D < PAIR_BOUND;
+    const int32_t slack = emu_rp_slack_override >= 0 ? emu_rp_slack_override : (int32_t)(PAIR_BOUND - D);
     const int H = n - L;
     const uint64_t lanes = 1ull << H;
     // the device flow: every lane's best block key, the (E, lane rank, block key) winner
@@ -60,6 +68,14 @@ int emu_i32(const int32_t* Q, int n, int B, int L, int64_t* E_out, uint64_t* ran
                                                              dlp.data());
             if (pk != key) return -5;
             ++emu_stat_pair_lanes;
+        }
+        if (rp) {
+            uint32_t fallbacks = 0;
+            const int64_t rk = lane_best_block_rp<M, OMAX>(W.data(), Wp.data(), d.data(), n, L, B, lam, dlp.data(),
+                                                           slack, &fallbacks);
+            if (rk != key) return -6;
+            ++emu_stat_rp_lanes;
+            emu_stat_rp_int32_groups += fallbacks;
         }
     }
     *E_out = bE;
@@ -128,7 +144,9 @@ extern "C" {
 
 // Returns 0 on success; -1 unsupported M; -2 unsupported L; -3 the two i32 flows disagree;
 // -4 float energies out of range; -5 the packed pair walk's lane key differs from the
-// int32 walk's (checked for every lane when the problem's bound selects the pair walk).
+// int32 walk's (checked for every lane when the problem's bound selects the pair walk);
+// -6 the rebased pair walk's lane key differs (checked for every lane when its guard
+// constant D < 15359).
 int qbx_emu_solve_i32(const int32_t* Q, int n, int B, int L, int M, int64_t* E, uint64_t* rank,
                       uint64_t* state) {
     int rc;
@@ -182,6 +200,16 @@ uint64_t qbx_emu_take_pair_lanes(void) {
     emu_stat_pair_lanes = 0;
     return v;
 }
+
+// rebased pair walk: lanes checked and groups walked in int32 since the last call
+void qbx_emu_take_rp_stats(uint64_t* lanes, uint64_t* int32_groups) {
+    *lanes = emu_stat_rp_lanes;
+    *int32_groups = emu_stat_rp_int32_groups;
+    emu_stat_rp_lanes = emu_stat_rp_int32_groups = 0;
+}
+
+// guard slack used by the rebased pair walk (-1: PAIR_BOUND - D, as the device uses)
+void qbx_emu_set_rp_slack(int32_t slack) { emu_rp_slack_override = slack; }
 
 uint64_t qbx_emu_state_to_rank(uint64_t s, int B) { return state_to_rank(s, B); }
 uint64_t qbx_emu_rank_to_state(uint64_t r, int B) { return rank_to_state(r, B); }

@@ -759,6 +759,309 @@ QBX_HD int64_t lane_best_block_pair(const int32_t* __restrict__ W, const uint32_
     return (int64_t)(((uint64_t)(uint32_t)E << 32) | (uint64_t)best.K);
 }
 
+// ------------------------------------------- rebased pair walk (energies relative to blocks)
+// Integer problems beyond PAIR_BOUND whose energy *differences inside a block* fit 16 bits
+// (random int16 QUBOs, for instance) walk the same two blocks per 32-bit word, with each
+// half holding E(y) - E(base of its block) + PAIR_BIAS; the int32 block bases E0 (pair bit
+// clear) and E0 + up (pair bit set) are kept beside them.  Inside a block,
+//     |E(y) - E(base)| <= sum_k |f_k| + sum_{k<l<M} |W_kl|,
+// and within one unrolled group of 16 blocks (walked bits 1..4 vary, as does nothing else)
+// every field stays within its value at the group's start plus the couplings to those 4 bits;
+// the pair bit adds |W_Mk| to the high half's fields.  So with
+//     D = sum_{k<l<M} |W_kl| + sum_k sum_{i<4} |W_{M+1+i,k}| + sum_k |W_Mk|   (per problem)
+// a group may run packed when sum_k |f_k(group start)| <= PAIR_BOUND - D (the guard, one
+// v_sad_u32 per field); every energy the packed group materialises then fits its half.
+// A wave any of whose lanes fails the guard walks that group in int32 (both halves, one after
+// the other, trie constants unpacked from the packed table) -- a rare, slower path with the
+// same keys.  Fields are kept unpacked between groups (exact, any size) and packed for the
+// group: F_k = f_k + (f_k + W_Mk) 2^16; at the group's end the guard has bounded every |f_k|
+// below 2^15, so f_k is the low half of F_k, sign-extended.
+template <int M, int OMAX>
+struct LaneRP {
+    int32_t E0;          // energy of the low block's base (pair bit clear)
+    int32_t up;          // field of the pair bit: the high block's base is E0 + up
+    int32_t f[M];        // inner fields of the low block (unpacked between groups)
+    int32_t u[OMAX];     // upper fields of walked bit q (state bit M + 1 + q), low block
+};
+
+// Row J = M+1+I of a packed group transition: the packed inner couplings and the unpacked
+// couplings to the pair bit (w[0]) and to walked bits 0 .. I-1 (w[1 + q]).
+template <int M>
+struct WRowRP {
+    uint32_t p[M];
+    int32_t w[4];
+};
+
+template <int M, int I>
+QBX_HD void load_wrow_rp(WRowRP<M>& r, const int32_t* __restrict__ W, const uint32_t* __restrict__ Wp) {
+    constexpr int J = M + 1 + I;
+    const uint32_t* WPJ = Wp + J * LDW;
+    const int32_t* WJ = W + J * LDW;
+#pragma unroll
+    for (int k = 0; k < M; ++k) r.p[k] = WPJ[k];
+    r.w[0] = WJ[M];
+#pragma unroll
+    for (int q = 0; q < I; ++q) r.w[1 + q] = WJ[M + 1 + q];
+}
+
+template <int M, int OMAX, int I>
+QBX_HD void trans_row_rp(LaneRP<M, OMAX>& s, uint32_t* F, const WRowRP<M>& r, bool up) {
+    const uint32_t field = (uint32_t)s.u[I] + (I > 0 ? (uint32_t)r.w[I] : 0u);   // w[I] = W[J][J-1]
+    if (up) {
+        s.E0 = (int32_t)((uint32_t)s.E0 + field);
+        s.up = (int32_t)((uint32_t)s.up + (uint32_t)r.w[0]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) F[k] += r.p[k];
+#pragma unroll
+        for (int q = 0; q < I; ++q) s.u[q] = (int32_t)((uint32_t)s.u[q] + (uint32_t)r.w[1 + q]);
+    } else {
+        s.E0 = (int32_t)((uint32_t)s.E0 - field);
+        s.up = (int32_t)((uint32_t)s.up - (uint32_t)r.w[0]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) F[k] -= r.p[k];
+#pragma unroll
+        for (int q = 0; q < I; ++q) s.u[q] = (int32_t)((uint32_t)s.u[q] - (uint32_t)r.w[1 + q]);
+    }
+}
+
+// unpacked transition on walked bit i (state bit J = M + 1 + i), any i (run-time)
+template <int M, int OMAX>
+QBX_HD void trans_dyn_rp(LaneRP<M, OMAX>& s, const int32_t* __restrict__ WJ, int i, bool up) {
+    const int J = M + 1 + i;
+    uint32_t uJ = 0;
+#pragma unroll
+    for (int q = 0; q < OMAX; ++q) if (q == i) uJ = (uint32_t)s.u[q];
+    const uint32_t field = uJ + (i > 0 ? (uint32_t)WJ[J - 1] : 0u);
+    const uint32_t sg = up ? 1u : 0xffffffffu;
+    s.E0 = (int32_t)((uint32_t)s.E0 + sg * field);
+    s.up = (int32_t)((uint32_t)s.up + sg * (uint32_t)WJ[M]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] = (int32_t)((uint32_t)s.f[k] + sg * (uint32_t)WJ[k]);
+#pragma unroll
+    for (int q = 0; q < OMAX; ++q) if (q < i) s.u[q] = (int32_t)((uint32_t)s.u[q] + sg * (uint32_t)WJ[M + 1 + q]);
+}
+
+// unpacked transition on walked bit i < GROUP_BITS (inside a group; run-time i)
+template <int M, int OMAX>
+QBX_HD void trans_grp_rp(LaneRP<M, OMAX>& s, const int32_t* __restrict__ WJ, int i, bool up) {
+    const int J = M + 1 + i;
+    uint32_t uJ = 0;
+#pragma unroll
+    for (int q = 0; q < GROUP_BITS; ++q) if (q == i) uJ = (uint32_t)s.u[q];
+    const uint32_t field = uJ + (i > 0 ? (uint32_t)WJ[J - 1] : 0u);
+    const uint32_t sg = up ? 1u : 0xffffffffu;
+    s.E0 = (int32_t)((uint32_t)s.E0 + sg * field);
+    s.up = (int32_t)((uint32_t)s.up + sg * (uint32_t)WJ[M]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] = (int32_t)((uint32_t)s.f[k] + sg * (uint32_t)WJ[k]);
+#pragma unroll
+    for (int q = 0; q + 1 < GROUP_BITS; ++q) if (q < i) s.u[q] = (int32_t)((uint32_t)s.u[q] + sg * (uint32_t)WJ[M + 1 + q]);
+}
+
+// Lane best-key update from the minima of the two blocks of packed block c (mA, mB: int32)
+QBX_HD void rp_best_update(int32_t mA, int32_t mB, uint32_t c, int SBk, uint32_t pmaskS, LaneBest& best) {
+    const int32_t m = mA < mB ? mA : mB;
+    if (__builtin_expect(m <= best.E, 0)) {
+        const uint32_t cf = (c << 1) | (c & 1u);   // int32-walk block of the low half
+        if (mA == m) {
+            const uint32_t K = block_key(cf, SBk, pmaskS);
+            if (m < best.E || K < best.K) { best.E = m; best.K = K; }
+        }
+        if (mB == m) {
+            const uint32_t K = block_key(cf ^ 1u, SBk, pmaskS);
+            if (m < best.E || K < best.K) { best.E = m; best.K = K; }
+        }
+    }
+}
+
+template <int M, int OMAX, int J>
+QBX_HD bool group_step_rp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ W,
+                          const uint32_t* __restrict__ Wp, const uint32_t* dl, uint32_t c0, uint32_t nblk,
+                          int SBk, uint32_t pmaskS, LaneBest& best, WRowRP<M>& row) {
+    const uint32_t c = c0 + (uint32_t)J;
+    if (J > 0) {
+        if (c >= nblk) return false;
+        constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
+        const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
+                                             : (((c0 >> GROUP_BITS) & 1u) == 0u);
+        trans_row_rp<M, OMAX, i>(s, F, row, up);
+    }
+    if (J + 1 < (1 << GROUP_BITS)) load_wrow_rp<M, __builtin_ctz((unsigned)(J + 1))>(row, W, Wp);
+    const uint32_t mp = block_min_pair<M>((uint32_t)PAIR_BIAS * 65537u, F, dl);
+    const int32_t b = s.E0 - PAIR_BIAS;
+    const int32_t mA = b + (int32_t)(mp & 0xffffu);
+    const int32_t mB = b + s.up + (int32_t)(mp >> 16);
+    rp_best_update(mA, mB, c, SBk, pmaskS, best);
+    return true;
+}
+
+template <int M, int OMAX, int... Js>
+QBX_HD void group_rp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ W, const uint32_t* __restrict__ Wp,
+                     const uint32_t* dl, uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best,
+                     std::integer_sequence<int, Js...>) {
+    WRowRP<M> row;
+    (void)(group_step_rp<M, OMAX, Js>(s, F, W, Wp, dl, c0, nblk, SBk, pmaskS, best, row) && ...);
+}
+
+// Unpacked int32 minimum of one block whose trie constants come packed (low half, exact:
+// every |c_y| <= sum_{k<l<M} |W_kl| < 2^15 on this path).  z is an opaque zero added before
+// each decode, so that the decoded constants are not hoisted out of the caller's loop (that
+// would keep 2^M more values live).
+QBX_HD int32_t unpack_lo(uint32_t v, uint32_t z) { return (int32_t)(int16_t)((v + z) & 0xffffu); }
+
+template <int M, unsigned P>
+QBX_HD void utrie_even(uint32_t EP, const int32_t* f, const uint32_t* cc, uint32_t z, int32_t& mn);
+
+template <int M, unsigned P, int K>
+QBX_HD void utrie_children(uint32_t EP, const int32_t* f, const uint32_t* cc, uint32_t z, int32_t& mn) {
+    if constexpr (K < M && K < ctz_c(P)) {
+        constexpr unsigned C = P | (1u << K);
+        utrie_even<M, C>(EP + (uint32_t)f[K] + (uint32_t)unpack_lo(cc[C], z), f, cc, z, mn);
+        utrie_children<M, P, K + 1>(EP, f, cc, z, mn);
+    }
+}
+
+template <int M, unsigned P>
+QBX_HD void utrie_even(uint32_t EP, const int32_t* f, const uint32_t* cc, uint32_t z, int32_t& mn) {
+    const uint32_t Eodd = EP + (uint32_t)f[0] + (uint32_t)unpack_lo(cc[P | 1u], z);
+    mn = min3_i32(mn, (int32_t)EP, (int32_t)Eodd);
+    utrie_children<M, P, 1>(EP, f, cc, z, mn);
+}
+
+// The group at c0 walked in int32 (the rare path of a wave whose guard failed): the low
+// blocks' state advances as in the packed group (and is left as the packed group leaves
+// it), and each block's high twin is evaluated from it (fields + W_Mk, base + up); blocks
+// and halves in run-time loops, so that the path stays small in code and registers.
+template <int M, int OMAX>
+QBX_HD void group_rp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ W, const uint32_t* dl,
+                           uint32_t c0, uint32_t nblk, int SBk, uint32_t pmaskS, LaneBest& best) {
+    const int32_t* WM = W + M * LDW;
+    const uint32_t cend = (c0 + (1u << GROUP_BITS) < nblk) ? c0 + (1u << GROUP_BITS) : nblk;
+#pragma unroll 1
+    for (uint32_t c = c0; c < cend; ++c) {
+        if (c > c0) {
+            const int i = ctz32(c - c0);
+            const bool up = (((c ^ (c >> 1)) >> i) & 1u) != 0u;
+            trans_grp_rp<M, OMAX>(s, W + (M + 1 + i) * LDW, i, up);
+        }
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            int32_t fh[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k) fh[k] = (int32_t)((uint32_t)s.f[k] + (h ? (uint32_t)WM[k] : 0u));
+            const int32_t E0h = (int32_t)((uint32_t)s.E0 + (h ? (uint32_t)s.up : 0u));
+            int32_t mn = E0h;
+            uint32_t z = 0u;
+            QBX_OPAQUE(z);
+            utrie_even<M, 0u>((uint32_t)E0h, fh, dl, z, mn);
+            if (mn <= best.E) {
+                const uint32_t K = block_key((c << 1) | ((c & 1u) ^ (uint32_t)h), SBk, pmaskS);
+                if (mn < best.E || K < best.K) { best.E = mn; best.K = K; }
+            }
+        }
+    }
+}
+
+// Set-up: the int32 walk's first block (pair bit clear) and the pair bit's field.
+template <int M, int OMAX>
+QBX_HD void lane_setup_rp(LaneRP<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* __restrict__ d,
+                          int n, int L, int B, uint64_t lam) {
+    LaneI32<M, OMAX> a;
+    lane_setup_i32<M, OMAX>(a, W, d, n, L, B, lam);
+    s.E0 = a.E0;
+    s.up = a.u[0];
+#pragma unroll
+    for (int k = 0; k < M; ++k) s.f[k] = a.f[k];
+#pragma unroll
+    for (int q = 0; q + 1 < OMAX; ++q) s.u[q] = a.u[q + 1];
+    s.u[OMAX - 1] = 0;
+}
+
+// The int32 walk's lane key (lane_best_block_i32) from the rebased pair walk.  Needs L > M,
+// every |c_y| < 2^15 and slack = PAIR_BOUND - D >= 0 (D as above, for this L and M).
+// groups_int32 (nullable) counts the groups walked in int32.
+template <int M, int OMAX>
+QBX_HD int64_t lane_best_block_rp(const int32_t* __restrict__ W, const uint32_t* __restrict__ Wp,
+                                  const int32_t* __restrict__ d, int n, int L, int B, uint64_t lam,
+                                  const uint32_t* dlp, int32_t slack, uint32_t* groups_int32) {
+    static_assert(OMAX > GROUP_BITS, "OMAX must cover the unrolled transition bits");
+    const int O = L - M;
+    const uint64_t p = lane_prefix(lam, L, B);
+    LaneRP<M, OMAX> s;
+    lane_setup_rp<M, OMAX>(s, W, d, n, L, B, lam);
+    const int SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
+    const uint32_t pmaskS = (parity64(p) && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
+    LaneBest best{0x7fffffff, 0xffffffffu};
+    const uint32_t nblk = 1u << (O - 1);
+    const int32_t* WM = W + M * LDW;
+    for (uint32_t c0 = 0;;) {
+        uint32_t g = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) g += (uint32_t)(s.f[k] < 0 ? -s.f[k] : s.f[k]);
+        bool ok = g <= (uint32_t)slack;
+#if defined(__HIP_DEVICE_COMPILE__)
+        ok = __all(ok);
+#endif
+        if (__builtin_expect(ok, 1)) {
+            uint32_t F[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k) F[k] = (uint32_t)s.f[k] + (((uint32_t)s.f[k] + (uint32_t)WM[k]) << 16);
+            group_rp<M, OMAX>(s, F, W, Wp, dlp, c0, nblk, SBk, pmaskS, best,
+                              std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
+#pragma unroll
+            for (int k = 0; k < M; ++k) s.f[k] = (int32_t)(int16_t)(F[k] & 0xffffu);
+        } else {
+            group_rp_int32<M, OMAX>(s, W, dlp, c0, nblk, SBk, pmaskS, best);
+            if (groups_int32) ++*groups_int32;
+        }
+        c0 += 1u << GROUP_BITS;
+        if (c0 >= nblk) break;
+        const int i = ctz32(c0);
+        const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
+        trans_dyn_rp<M, OMAX>(s, W + (M + 1 + i) * LDW, i, up);
+    }
+    return (int64_t)(((uint64_t)(uint32_t)best.E << 32) | (uint64_t)best.K);
+}
+
+// D of the guard for a problem (bit-ordered W), inner bits M, walk bits L.
+QBX_HD int64_t rp_guard_D(const int32_t* W, int M, int L) {
+    int64_t cin = 0, dg = 0, dp = 0;
+    for (int k = 0; k < M; ++k)
+        for (int l = k + 1; l < M; ++l) {
+            const int64_t w = W[k * LDW + l];
+            cin += w < 0 ? -w : w;
+        }
+    for (int k = 0; k < M; ++k) {
+        const int64_t wm = W[M * LDW + k];
+        dp += wm < 0 ? -wm : wm;
+        for (int i = 0; i < GROUP_BITS; ++i) {
+            const int J = M + 1 + i;
+            if (J >= L) break;
+            const int64_t w = W[J * LDW + k];
+            dg += w < 0 ? -w : w;
+        }
+    }
+    return cin + dg + dp;
+}
+
+// Whether the rebased pair walk is expected to pay off: the guard's room PAIR_BOUND - D
+// against 1.5 x sum_k sqrt(E f_k^2), the fields' root mean square over the states the
+// walk visits (bits M .. n-1 uniform).  Only the speed depends on this choice.
+QBX_HD bool rp_worth_it(const int32_t* W, const int32_t* d, int n, int M, int L, int64_t D) {
+    if (D >= PAIR_BOUND || L <= M) return false;
+    double S = 0.0;
+    for (int k = 0; k < M; ++k) {
+        double mean = d[k], var = 0.0;
+        for (int j = M; j < n; ++j) {
+            const double w = W[k * LDW + j];
+            mean += 0.5 * w;
+            var += 0.25 * w * w;
+        }
+        S += sqrt(mean * mean + var);
+    }
+    return (double)D + 1.5 * S <= (double)PAIR_BOUND;
+}
+
 // ----------------------------------------------- float problems: int32 filter, FP64 walk
 // The block filter runs on quantized tables Wq = rint(W 2^s), dq = rint(d 2^s) in exact
 // int32 arithmetic (the int walk above, unchanged).  Each of the terms d_a, W_ab that a

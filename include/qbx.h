@@ -60,17 +60,20 @@ typedef struct {
     int32_t inner_bits;
     int32_t lane_bits;
     int32_t status;       /* 0 = ok, QBX_ERANGE = energy bound outside the int32 path */
-    int32_t path;         /* walk that ran: QBX_PATH_INT32, QBX_PATH_PAIR16 or QBX_PATH_FLOAT */
+    int32_t path;         /* walk that ran: QBX_PATH_* */
     int32_t reserved;
 } qbx_result_t;
 
 /* qbx_result_t.path.  Integer problems whose energy bound (sum |Q_ii| + sum_{i<j} |Q_ij + Q_ji|)
- * is at most QBX_PAIR_BOUND walk two states per 32-bit operation (16-bit energies, same
- * answers); other integer problems walk in int32; float problems take the quantized filter
- * with FP64 re-walks. */
+ * is at most QBX_PAIR_BOUND walk two states per 32-bit operation (16-bit energies:
+ * QBX_PATH_PAIR16); other integer problems whose energy differences inside a walk block
+ * are expected to fit 16 bits do the same relative to each block's base, with a per-group
+ * guard and int32 groups where it fails (QBX_PATH_PAIR16_REBASED); the rest walk in int32.
+ * All give the same answers.  Float problems take the quantized filter with FP64 re-walks. */
 #define QBX_PATH_INT32 0
 #define QBX_PATH_PAIR16 1
 #define QBX_PATH_FLOAT 2
+#define QBX_PATH_PAIR16_REBASED 3
 #define QBX_PAIR_BOUND 15359
 
 typedef struct {

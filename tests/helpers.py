@@ -90,3 +90,17 @@ def pair_case(rng, n, trial):
     gap = PAIR_BOUND - energy_bound(Q)
     Q[0, 0] += -gap if Q[0, 0] < 0 or kind == 2 else gap
     return Q
+
+
+def guard_failing_case(rng, n, w=600, noise=3):
+    """An integer problem beyond the pair bound that takes the rebased pair walk and whose
+    guard fails in 1/16 of the groups: every inner field (state bits 0..6, variables
+    n-1..n-7) is w times the number of set bits among state bits 12..15, so the fields' sum
+    reaches 28 w when all four are set, beyond the guard's room; small noise elsewhere."""
+    Q = rng.integers(-noise, noise + 1, (n, n))
+    Q = np.triu(Q) + np.triu(Q, 1).T
+    for a in range(12, 16):
+        for k in range(7):
+            i, j = n - 1 - a, n - 1 - k
+            Q[i, j] = Q[j, i] = w // 2     # W[a][k] = Q_ij + Q_ji = w (w even)
+    return Q

@@ -36,7 +36,7 @@ def test_pair_walk_vs_oracle(gpu_ctx, inner):
 
 
 def test_pair_range_edges(gpu_ctx):
-    """Energies at exactly -15359 and +15359, the bound just above the pair range (int32
+    """Energies at exactly -15359 and +15359, the bound just above the pair range (another
     walk), and no outer bit to pair on (int32 walk)."""
     n = 20
     rng = np.random.default_rng(5)
@@ -52,7 +52,7 @@ def test_pair_range_edges(gpu_ctx):
     over = low.copy()
     over[0, 0] -= 1
     got = engine.solve_matrix(over)
-    assert got.info["walk"] == "int32" and (got.E, got.state) == oracle.brute_i64(over)
+    assert got.info["walk"] != "pair16" and (got.E, got.state) == oracle.brute_i64(over)
     Q = random_instance(rng, 12, "ternary")
     got = engine.solve_matrix(Q, walk_bits=7, inner_bits=7)
     assert got.info["walk"] == "int32" and (got.E, got.state) == oracle.brute_i64(Q)
@@ -60,11 +60,12 @@ def test_pair_range_edges(gpu_ctx):
 
 def test_pair_matches_int32_walk_large(gpu_ctx):
     """n = 30 .. 34 MaxCut and ternary problems: the pair walk's answer equals the int32
-    walk's on the same problem scaled by k (same argmin, energies x k, bound out of range)."""
+    walk's on the same problem scaled by k = 4096 (same argmin, energies x k; couplings too
+    large for either packed walk)."""
     rng = np.random.default_rng(6)
     for n, kind in ((30, "maxcut"), (32, "ternary"), (34, "maxcut")):
         Q = random_instance(rng, n, kind)
-        k = PAIR_BOUND // energy_bound(Q) + 1
+        k = 4096
         for B in (None, 0):
             flush("pair large", n, kind, B, k)
             a = engine.solve_matrix(Q, tie_B=B)
@@ -89,3 +90,57 @@ def test_pair_in_batches_and_shares(gpu_ctx):
     full = engine.solve_matrix(Q, walk_bits=16)
     merged = engine.merge_parts([engine.solve_part(Q, k, 8, walk_bits=16) for k in range(8)])
     assert (merged.E, merged.state) == (full.E, full.state)
+
+
+def sym_int16(rng, n, scale=127):
+    U = rng.integers(-scale, scale + 1, (n, n))
+    return np.triu(U) + np.triu(U, 1).T
+
+
+@pytest.mark.parametrize("inner", [7, 6, 4, 1])
+def test_rebased_pair_walk_vs_oracle(gpu_ctx, inner):
+    """Random int16-range QUBOs (bounds far beyond 15359) take the rebased pair walk:
+    16-bit energies relative to each block's base."""
+    rng = np.random.default_rng(950 + inner)
+    omax = {7: 13, 6: 14, 4: 16, 1: 19}[inner]
+    for trial in range(10):
+        n = int(rng.integers(max(inner + 2, 16), 23))
+        Q = sym_int16(rng, n)
+        while energy_bound(Q) <= PAIR_BOUND:     # beyond the plain pair walk's range
+            Q = sym_int16(rng, n)
+        B = [None, 0, int(rng.integers(0, n + 1))][trial % 3]
+        ref = oracle.brute_i64(Q, B)
+        for L in sorted({inner + 1, min(n, inner + 5), min(n, inner + omax)}):
+            flush("rebased", inner, trial, n, B, L)
+            got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=inner)
+            assert (got.E, got.state) == ref, (inner, trial, n, B, L)
+            assert got.info["walk"] == "pair16_rebased", (inner, trial, n, L)
+
+
+def test_rebased_pair_guard_failures(gpu_ctx):
+    """Problems whose guard fails in 1/16 of the groups (those groups run in int32 on the
+    device, wave by wave): answers stay the oracle's."""
+    from helpers import guard_failing_case
+    rng = np.random.default_rng(951)
+    for n in (18, 20, 22):
+        Q = guard_failing_case(rng, n)
+        for B in (None, 0, 13):
+            flush("guard failures", n, B)
+            got = engine.solve_matrix(Q, tie_B=B, walk_bits=16)
+            assert got.info["walk"] == "pair16_rebased"
+            assert (got.E, got.state) == oracle.brute_i64(Q, B), (n, B)
+
+
+def test_rebased_matches_int32_walk_large(gpu_ctx):
+    """n = 30 .. 36 int16-range QUBOs: the rebased pair walk's answer equals the int32
+    walk's on the same problem scaled by 3 (same argmin, energies x3; the scaled couplings
+    leave the guard no room, so it walks in int32)."""
+    rng = np.random.default_rng(952)
+    for n in (30, 33, 36):
+        Q = sym_int16(rng, n)
+        for B in (None, 0):
+            flush("rebased large", n, B)
+            a = engine.solve_matrix(Q, tie_B=B)
+            b = engine.solve_matrix(Q * 3, tie_B=B)
+            assert a.info["walk"] == "pair16_rebased" and b.info["walk"] == "int32", (a.info["walk"], b.info["walk"])
+            assert (a.E * 3, a.state) == (b.E, b.state), (n, B)

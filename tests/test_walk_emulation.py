@@ -168,3 +168,63 @@ def test_pair_walk_matches_int32_walk(emu, M):
     Q = random_instance(rng, 9, "ternary")
     assert run_i32(emu, Q, 9, M, M) == oracle.brute_i64(Q, 9)
     assert emu.qbx_emu_take_pair_lanes() == 0
+
+
+def take_rp(emu):
+    lanes, groups = ctypes.c_uint64(), ctypes.c_uint64()
+    emu.qbx_emu_take_rp_stats(ctypes.byref(lanes), ctypes.byref(groups))
+    return lanes.value, groups.value
+
+
+def _rp_case(rng, n, trial):
+    """Integer problems beyond the pair bound whose in-block energy differences mostly fit
+    16 bits: random int16-range QUBOs of several scales."""
+    scale = [127, 300, 600, 1200][trial % 4]
+    U = rng.integers(-scale, scale + 1, (n, n))
+    return np.triu(U) + np.triu(U, 1).T
+
+
+@pytest.mark.parametrize("M", [1, 4, 6, 7])
+@pytest.mark.parametrize("slack", [-1, 0, 3000])
+def test_rebased_pair_walk_matches_int32_walk(emu, M, slack):
+    """The rebased pair walk (energies relative to each block's base, a per-group guard on
+    the fields, int32 groups where it fails) gives every lane the int32 walk's key.  slack
+    -1 is the device's guard; 0 forces every group of a lane whose fields are not all zero
+    into int32; 3000 mixes both within lanes."""
+    emu.qbx_emu_set_rp_slack.argtypes = [ctypes.c_int32]
+    emu.qbx_emu_set_rp_slack(slack)
+    take_rp(emu)
+    lanes = groups = 0
+    try:
+        rng = np.random.default_rng(700 + M + slack)
+        for trial in range(16):
+            n = int(rng.integers(M + 2, 16))
+            Q = _rp_case(rng, n, trial) if trial % 4 != 3 else pair_case(rng, n, trial)
+            B = [min(n, 20), 0, int(rng.integers(0, n + 1))][trial % 3]
+            ref = oracle.brute_i64(Q, B)
+            for L in sorted({M + 1, min(n, M + 5), min(n, M + 6), n}):
+                assert run_i32(emu, Q, B, L, M) == ref, (n, B, L, M, trial)
+                a, g = take_rp(emu)
+                lanes, groups = lanes + a, groups + g
+    finally:
+        emu.qbx_emu_set_rp_slack(-1)
+    assert lanes > 0
+    if slack >= 0:
+        assert groups > 0       # the int32 groups ran
+
+
+def test_rebased_pair_guard_fails_sometimes(emu):
+    """A problem the device sends down the rebased pair walk (its heuristic accepts it) whose
+    guard fails in some groups: the int32 groups run under the device's own slack, and the
+    answers stay exact."""
+    from helpers import guard_failing_case
+    rng = np.random.default_rng(77)
+    take_rp(emu)
+    for n in (17, 18):
+        Q = guard_failing_case(rng, n)
+        assert energy_bound(Q) > PAIR_BOUND
+        ref = oracle.brute_i64(Q, None)
+        assert run_i32(emu, Q, min(n, 20), 16, 7) == ref
+        lanes, groups = take_rp(emu)
+        total = lanes * (2 ** (16 - 7 - 1) // 16)
+        assert lanes > 0 and 0 < groups < total, (lanes, groups, total)
