@@ -46,12 +46,12 @@ def test_pair_range_edges(gpu_ctx):
     high[0, 0] += PAIR_BOUND - energy_bound(high)
     for Q, walk in ((low, "pair16"), (high, "pair16"), (-high, "pair16")):
         assert energy_bound(Q) == PAIR_BOUND
-        got = engine.solve_matrix(Q)
+        got = engine.solve_matrix(Q, walk_bits=12)
         assert (got.E, got.state) == oracle.brute_i64(Q), walk
         assert got.info["walk"] == walk
     over = low.copy()
     over[0, 0] -= 1
-    got = engine.solve_matrix(over)
+    got = engine.solve_matrix(over, walk_bits=12)
     assert got.info["walk"] != "pair16" and (got.E, got.state) == oracle.brute_i64(over)
     Q = random_instance(rng, 12, "ternary")
     got = engine.solve_matrix(Q, walk_bits=7, inner_bits=7)
