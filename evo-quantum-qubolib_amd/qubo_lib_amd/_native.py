@@ -369,7 +369,9 @@ class Context:
         return packed, offsets, ns
 
     def batch(self, Qs, kind: str = "i32", packed=None, **kw):
-        """Solve a list of problems in one persistent launch; returns a list of Result."""
+        """Solve a list of problems in one persistent launch; returns a sequence of Result
+        (the ctypes array the library filled: indexing and iteration give views of it, so
+        no per-result Python objects are made until they are used)."""
         dtype, fn, ptr = ((np.int32, self.lib.qbx_batch_i32, ctypes.c_int32) if kind == "i32"
                           else (np.float64, self.lib.qbx_batch_f64, ctypes.c_double))
         if packed is None:
@@ -383,7 +385,7 @@ class Context:
                     ns.ctypes.data_as(_P(ctypes.c_int32)), ctypes.byref(o), out)
         if rc not in (QBX_OK, QBX_ERANGE):
             _check(self.lib, rc)
-        return list(out)
+        return out
 
     # -- RandomQUBO batches drawn on the device ----------------------------------------
     @staticmethod
@@ -404,7 +406,7 @@ class Context:
 
     def random_qubo_batch(self, seeds, ns, low=-100, high=100, dtype=np.float32, **kw):
         """Solve RandomQUBO(seed, N, low, high, dtype) instances whose matrices are drawn on
-        the device (bit for bit numpy's); returns a list of Result."""
+        the device (bit for bit numpy's); returns a sequence of Result (as batch())."""
         states, nsa, code = self._gen_args(seeds, ns, dtype)
         out = (Result * len(seeds))()
         o = _opts(**kw)
@@ -413,7 +415,7 @@ class Context:
                                                 int(low), int(high), code, ctypes.byref(o), out)
         if rc not in (QBX_OK, QBX_ERANGE):
             _check(self.lib, rc)
-        return list(out)
+        return out
 
     def random_qubo_read(self, seeds, ns, low=-100, high=100, dtype=np.float32) -> list:
         """The matrices random_qubo_batch would solve, drawn on the device and copied back
@@ -456,7 +458,7 @@ class BatchPlan:
 
     def run(self, Qs=None, packed=None):
         """Solve matrices of the planned sizes (a list, or Context.pack output); returns a
-        list of Result."""
+        sequence of Result (as Context.batch)."""
         if packed is None:
             packed = Context.pack(Qs, self.dtype)
         data, offsets, ns = packed
@@ -471,7 +473,7 @@ class BatchPlan:
             rc = fn(self._h, data.ctypes.data_as(_P(ptr)), offsets.ctypes.data_as(_P(ctypes.c_int64)), out)
         if rc not in (QBX_OK, QBX_ERANGE):
             _check(self.ctx.lib, rc)
-        return list(out)
+        return out
 
     def close(self):
         if self._h:
