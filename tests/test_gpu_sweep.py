@@ -38,3 +38,32 @@ def test_random_batches(gpu_ctx):
         Qs = [random_instance(rng, int(rng.integers(1, 21)), KINDS[i % len(KINDS)]) for i in range(200)]
         for Q, r in zip(Qs, engine.solve_matrices(Qs)):
             assert (r.E, r.state) == oracle.brute_i64(Q), (rep, Q.shape)
+
+
+def test_packed_walks_sweep(gpu_ctx):
+    """The three integer walks side by side: int16-range QUBOs of random scale (bounds from
+    a few thousand, the plain pair walk, to far beyond the rebased walk's reach, the int32
+    walk), problems whose guard fails in some groups, random tie widths, variants and walk
+    widths; every answer against the oracle, and every walk taken at least 20 times."""
+    from helpers import guard_failing_case
+    rng = np.random.default_rng(2026)
+    seen = {}
+    for trial in range(600):
+        n = int(rng.integers(8, 23))
+        if trial % 10 == 9:
+            n = max(n, 17)
+            Q = guard_failing_case(rng, n)
+        else:
+            scale = int(rng.choice([3, 20, 60, 127, 300, 2000, 30000]))
+            U = rng.integers(-scale, scale + 1, (n, n))
+            Q = np.triu(U) + np.triu(U, 1).T if trial % 2 else U
+        B = int(rng.integers(0, n + 1)) if trial % 3 else None
+        cands = [v for v in VARIANTS if v[0] < n]
+        M, OMAX = cands[trial % len(cands)]
+        L = int(rng.integers(M, min(n, M + OMAX) + 1))
+        if trial % 10 == 9:
+            L = max(L, min(n, 16))
+        got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=M)
+        assert (got.E, got.state) == oracle.brute_i64(Q, B), ("packed sweep", trial, n, B, L, M, got.info["walk"])
+        seen[got.info["walk"]] = seen.get(got.info["walk"], 0) + 1
+    assert all(seen.get(w, 0) >= 20 for w in ("int32", "pair16", "pair16_rebased")), seen
