@@ -44,3 +44,10 @@ def test_no_cpu_fallback_without_gpu():
     from qubo_lib_amd.problems import RandomQUBO
     with pytest.raises(_native.NativeError):
         QBF().solve(RandomQUBO.init_random_instance(seed=0, N=8, dtype="int32"))
+
+
+def test_path_codes_match_header():
+    text = HEADER.read_text()
+    codes = {int(v): k for k, v in re.findall(r"#define QBX_PATH_([A-Z0-9_]+)\s+(\d+)", text)}
+    assert {c: name.lower() for c, name in codes.items()} == _native.PATHS
+    assert int(re.search(r"#define QBX_PAIR_BOUND\s+(\d+)", text).group(1)) == _native.PAIR_BOUND
