@@ -381,7 +381,7 @@ def serve(args, ctx, devinfo):
                      "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "wall_s": wall,
                      "steps": int(st["steps"]), "device_busy_ms": st["busy_ms"]})
         log(f"serve rate={rate:.0f}/s -> {rows[-1]}")
-    # capacity: every request submitted at once (the client's submit loop is the only pacing)
+    # capacity: every request submitted at once, in one submit_many call
     with Server(max_n=40) as srv:
         for p in preps[:64]:
             srv.submit(p)
@@ -389,7 +389,7 @@ def serve(args, ctx, devinfo):
         while w < 64:
             w += len(srv.poll(wait_ms=20))
         t0 = time.perf_counter()
-        ids = [srv.submit(p) for p in preps]
+        ids = srv.submit_many(preps)
         t_sub = time.perf_counter() - t0
         got = {}
         while len(got) < len(ids):

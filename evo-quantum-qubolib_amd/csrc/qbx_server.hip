@@ -472,32 +472,73 @@ void server_free(qbx_server* S) {
     if (S->d_results) (void)hipFree(S->d_results);
 }
 
+// A request for problem Q of size n, planned and copied (not yet queued).
 template <class T>
-int submit(qbx_server* S, int kind, const T* Q, int n, int tie_B, uint64_t* id) {
-    if (!S || !Q || !id) return set_err(QBX_EINVAL, "null argument");
-    if (n < 1 || n > S->opts.max_n) return set_err(QBX_EINVAL, "n=%d outside [1, %d] (server max_n)", n, S->opts.max_n);
+int make_request(qbx_server* S, int kind, const T* Q, int n, int tie_B, int index, Req** out) {
+    if (n < 1 || n > S->opts.max_n)
+        return set_err(QBX_EINVAL, "request %d: n=%d outside [1, %d] (server max_n)", index, n, S->opts.max_n);
     Req* r = new Req();
     r->kind = kind;
     r->n = n;
     r->tie_B = tie_B;
     r->work = ldexp(1.0, n);
     qbx_opts_t o = {tie_B, S->opts.walk_bits, -1, 0};
-    const int rc = plan_problem(kind, n, o, PLAN_BATCH, 0, r->plan, &S->walk);
+    const int rc = plan_problem(kind, n, o, PLAN_BATCH, index, r->plan, &S->walk);
     if (rc != QBX_OK) { delete r; return rc; }
-    if (r->plan.chunks > S->L.max_chunks) { delete r; return set_err(QBX_EINVAL, "walk_bits too small for the slot size"); }
+    if (r->plan.chunks > S->L.max_chunks) {
+        delete r;
+        return set_err(QBX_EINVAL, "request %d: walk_bits too small for the slot size", index);
+    }
     r->Q.resize(sizeof(T) * (size_t)n * n);
     memcpy(r->Q.data(), Q, r->Q.size());
+    *out = r;
+    return QBX_OK;
+}
+
+// Queue requests (ids in order, under one lock, one wake-up of the worker), or none.
+int enqueue_requests(qbx_server* S, Req** reqs, int count, uint64_t* ids) {
     {
         std::lock_guard<std::mutex> g(S->m);
-        if (S->stop) { delete r; return set_err(QBX_EBUSY, "server is stopping"); }
-        r->id = S->next_id++;
-        r->t_submit = S->now_ms();
-        *id = r->id;
-        S->incoming.push_back(r);
+        if (S->stop) {
+            for (int i = 0; i < count; ++i) delete reqs[i];
+            return set_err(QBX_EBUSY, "server is stopping");
+        }
+        const double t = S->now_ms();
+        for (int i = 0; i < count; ++i) {
+            reqs[i]->id = S->next_id++;
+            reqs[i]->t_submit = t;
+            ids[i] = reqs[i]->id;
+            S->incoming.push_back(reqs[i]);
+        }
     }
-    S->n_submitted++;
+    S->n_submitted += count;
     S->cv_in.notify_one();
     return QBX_OK;
+}
+
+template <class T>
+int submit(qbx_server* S, int kind, const T* Q, int n, int tie_B, uint64_t* id) {
+    if (!S || !Q || !id) return set_err(QBX_EINVAL, "null argument");
+    Req* r = nullptr;
+    const int rc = make_request<T>(S, kind, Q, n, tie_B, 0, &r);
+    if (rc != QBX_OK) return rc;
+    return enqueue_requests(S, &r, 1, id);
+}
+
+template <class T>
+int submit_many(qbx_server* S, int kind, int count, const T* Qpacked, const int64_t* offsets, const int32_t* ns,
+                int tie_B, uint64_t* ids) {
+    if (!S || count < 0 || (count > 0 && (!Qpacked || !offsets || !ns || !ids)))
+        return set_err(QBX_EINVAL, "null argument");
+    std::vector<Req*> reqs((size_t)count, nullptr);
+    for (int i = 0; i < count; ++i) {
+        const int rc = make_request<T>(S, kind, Qpacked + offsets[i], ns[i], tie_B, i, &reqs[i]);
+        if (rc != QBX_OK) {
+            for (int j = 0; j < i; ++j) delete reqs[j];
+            return rc;
+        }
+    }
+    return count ? enqueue_requests(S, reqs.data(), count, ids) : QBX_OK;
 }
 
 }  // namespace
@@ -544,6 +585,16 @@ int qbx_server_submit_i32(qbx_server_t* s, const int32_t* Q, int n, int tie_B, u
 
 int qbx_server_submit_f64(qbx_server_t* s, const double* Q, int n, int tie_B, uint64_t* id) {
     return submit<double>(s, KIND_F64, Q, n, tie_B, id);
+}
+
+int qbx_server_submit_many_i32(qbx_server_t* s, int count, const int32_t* Qpacked, const int64_t* offsets,
+                               const int32_t* ns, int tie_B, uint64_t* ids) {
+    return submit_many<int32_t>(s, KIND_I32, count, Qpacked, offsets, ns, tie_B, ids);
+}
+
+int qbx_server_submit_many_f64(qbx_server_t* s, int count, const double* Qpacked, const int64_t* offsets,
+                               const int32_t* ns, int tie_B, uint64_t* ids) {
+    return submit_many<double>(s, KIND_F64, count, Qpacked, offsets, ns, tie_B, ids);
 }
 
 int qbx_server_poll(qbx_server_t* S, qbx_request_result_t* out, int max, int wait_ms, int* count) {

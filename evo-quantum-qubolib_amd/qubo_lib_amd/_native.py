@@ -178,6 +178,10 @@ _SIGNATURES = {
                                              _P(ctypes.c_uint64)]),
     "qbx_server_submit_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, ctypes.c_int,
                                              _P(ctypes.c_uint64)]),
+    "qbx_server_submit_many_i32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_int32), _P(ctypes.c_int64),
+                                                  _P(ctypes.c_int32), ctypes.c_int, _P(ctypes.c_uint64)]),
+    "qbx_server_submit_many_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
+                                                  _P(ctypes.c_int32), ctypes.c_int, _P(ctypes.c_uint64)]),
     "qbx_server_poll": (ctypes.c_int, [ctypes.c_void_p, _P(RequestResult), ctypes.c_int, ctypes.c_int,
                                        _P(ctypes.c_int)]),
     "qbx_server_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ServerStats)]),

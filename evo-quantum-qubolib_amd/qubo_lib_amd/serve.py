@@ -53,6 +53,33 @@ class Server:
         self._kinds[rid.value] = p
         return rid.value
 
+    def submit_many(self, problems, tie_B=None) -> list:
+        """Queue a burst of problems with one library call per arithmetic path; returns
+        their request ids in input order."""
+        preps = []
+        for problem in problems:
+            Q = problem if isinstance(problem, (np.ndarray, engine.Prepared)) else problem.build_Q_matrix(use_gpu=False)
+            p = Q if isinstance(Q, engine.Prepared) else engine.prepare(Q)
+            if p.n > self.max_n:
+                raise ValueError(f"N={p.n} exceeds the server's max_n={self.max_n}")
+            preps.append(p)
+        tb = -1 if tie_B is None else int(tie_B)
+        ids = [0] * len(preps)
+        for kind, dtype, fn, ptr in (("i32", np.int32, self.lib.qbx_server_submit_many_i32, ctypes.c_int32),
+                                     ("f64", np.float64, self.lib.qbx_server_submit_many_f64, ctypes.c_double)):
+            idx = [i for i, p in enumerate(preps) if p.kind == kind]
+            if not idx:
+                continue
+            data, offsets, ns = _native.Context.pack([preps[i].matrix for i in idx], dtype)
+            out = np.zeros(len(idx), dtype=np.uint64)
+            _native._check(self.lib, fn(self._h, len(idx), data.ctypes.data_as(_P(ptr)),
+                                        offsets.ctypes.data_as(_P(ctypes.c_int64)), ns.ctypes.data_as(_P(ctypes.c_int32)),
+                                        tb, out.ctypes.data_as(_P(ctypes.c_uint64))))
+            for i, rid in zip(idx, out.tolist()):
+                ids[i] = rid
+                self._kinds[rid] = preps[i]
+        return ids
+
     def poll(self, wait_ms: int = 0, max_results: int = 1024) -> list:
         """Completed requests as dicts: id, E (int / float as the problem), state, rank,
         status, n and the server-clock times t_submit_ms / t_admit_ms / t_done_ms."""

@@ -221,6 +221,12 @@ int qbx_server_start(qbx_ctx_t *ctx, const qbx_server_opts_t *opts, qbx_server_t
 /* Copies Q; tie_B = -1 selects min(n, 20).  Never blocks on the GPU. */
 int qbx_server_submit_i32(qbx_server_t *s, const int32_t *Q, int n, int tie_B, uint64_t *id);
 int qbx_server_submit_f64(qbx_server_t *s, const double *Q, int n, int tie_B, uint64_t *id);
+/* A burst of requests in one call (packed as for qbx_batch_*): ids[i] for problem i, in
+ * order; all are queued, or none (the error names the first bad request). */
+int qbx_server_submit_many_i32(qbx_server_t *s, int count, const int32_t *Qpacked, const int64_t *offsets,
+                               const int32_t *ns, int tie_B, uint64_t *ids);
+int qbx_server_submit_many_f64(qbx_server_t *s, int count, const double *Qpacked, const int64_t *offsets,
+                               const int32_t *ns, int tie_B, uint64_t *ids);
 /* Moves up to `max` completed results into out[]; waits up to wait_ms for the first one. */
 int qbx_server_poll(qbx_server_t *s, qbx_request_result_t *out, int max, int wait_ms, int *count);
 int qbx_server_stats(qbx_server_t *s, qbx_server_stats_t *out);

@@ -109,3 +109,24 @@ def test_server_stop_with_requests_pending(gpu_ctx):
         srv.submit(rng.integers(-50, 50, (26, 26)))
     srv.close()
     assert time.time() - t0 < 60
+
+
+def test_submit_many_matches_individual_solves(gpu_ctx):
+    """A burst submitted in one call per arithmetic path (int and float mixed): ids come
+    back in input order, every answer equals a plain solve; a bad request queues nothing."""
+    rng = np.random.default_rng(5)
+    Qs = [random_instance(rng, int(rng.integers(1, 25)), KINDS[i % len(KINDS)]) for i in range(300)]
+    Qs.insert(7, rng.standard_normal((12, 12)))
+    Qs.insert(100, rng.standard_normal((19, 19)).astype(np.float32))
+    with Server(max_n=30, max_inflight=64) as srv:
+        ids = srv.submit_many(Qs)
+        assert len(set(ids)) == len(Qs)
+        int_ids = [i for q, i in zip(Qs, ids) if np.issubdtype(q.dtype, np.integer)]
+        assert int_ids == sorted(int_ids)
+        got = drain(srv, len(ids))
+        with pytest.raises(ValueError):
+            srv.submit_many([random_instance(rng, 10, "uniform"), random_instance(rng, 31, "uniform")])
+        assert srv.stats()["submitted"] == len(Qs)
+    for q, i in zip(Qs, ids):
+        alone = engine.solve_matrix(q)
+        assert got[i]["status"] == 0 and (got[i]["E"], got[i]["state"]) == (alone.E, alone.state), q.shape
