@@ -362,15 +362,16 @@ class Context:
     # -- batches --------------------------------------------------------------------
     @staticmethod
     def pack(Qs, dtype):
+        """(packed data, offsets, ns) of square matrices, row-major, back to back."""
         ns = np.array([q.shape[0] for q in Qs], dtype=np.int32)
         sizes = ns.astype(np.int64) ** 2
         offsets = np.zeros(len(Qs), dtype=np.int64)
         if len(Qs) > 1:
             offsets[1:] = np.cumsum(sizes)[:-1]
-        packed = np.empty(int(sizes.sum()), dtype=dtype)
-        for q, o, s in zip(Qs, offsets, sizes):
-            packed[o:o + s] = np.asarray(q, dtype=dtype).ravel()
-        return packed, offsets, ns
+        if not len(Qs):
+            return np.empty(0, dtype=dtype), offsets, ns
+        packed = np.concatenate([np.asarray(q).reshape(-1) for q in Qs]).astype(dtype, copy=False)
+        return np.ascontiguousarray(packed), offsets, ns
 
     def batch(self, Qs, kind: str = "i32", packed=None, **kw):
         """Solve a list of problems in one persistent launch; returns a sequence of Result
