@@ -20,8 +20,8 @@ def test_header_compiles_for_c_and_cpp(cmd, tmp_path):
     assert r.returncode == 0, r.stderr
 
 
-def _lcg_matrix(n=12):
-    x, out = 12345, []
+def _lcg_matrix(n=12, seed=12345):
+    x, out = seed, []
     for _ in range(n * n):
         x = (x * 1103515245 + 12345) & 0xFFFFFFFF
         out.append(((x >> 16) % 41) - 20)
@@ -39,3 +39,6 @@ def test_c_program_matches_python_binding(gpu_ctx):
     b = gpu_ctx.solve_i32(-Q)
     assert lines["solve"].split() == [str(a.E_int), str(a.state)]
     assert lines["batch"].split() == [str(a.E_int), str(a.state), str(b.E_int), str(b.state)]
+    assert lines["server"].split() == lines["batch"].split()
+    c = gpu_ctx.solve_i32(_lcg_matrix(20, 777))
+    assert lines["parts"].split() == [str(c.E_int), str(c.state)]
