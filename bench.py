@@ -179,7 +179,8 @@ def exhaustive(args, ctx, devinfo):
                                                        "walk": ref.info["walk"], "kernel_ms": ref.info["kernel_ms"]},
                       matches_int32_walk=bool(ref.E == E * k and ref.state == state))
     gold = ROOT / "tests" / "golden" / "oracle_goldens.json"
-    key = {"float": f"sym_normal_n34_s{args.seed}", "maxcut": None}.get(args.workload, f"sym_int16_n40_s{args.seed}")
+    key = {"float": f"sym_normal_n34_s{args.seed}", "maxcut": f"maxcut_n40_s{args.seed}"}.get(
+        args.workload, f"sym_int16_n40_s{args.seed}")
     if gold.exists() and key:
         g = json.loads(gold.read_text()).get(key)
         if g is not None:

@@ -171,6 +171,9 @@ def _golden_problem(name):
         return TSP.init_random_instance(seed=42, n_cities=6).build_Q_matrix()
     if name == "random_n37_seed42_f32":
         return RandomQUBO.init_random_instance(seed=42, N=37).build_Q_matrix()
+    if name.startswith("maxcut_n40_s"):
+        from qubo_lib_amd.problems.generators import maxcut_gnp
+        return maxcut_gnp(40, 0.5, seed=int(name[-1]))
     raise KeyError(name)
 
 
