@@ -58,10 +58,12 @@ struct Layout {
     size_t total_bytes = 0;
     // one part of a single problem's chunks (rank-space shard), or all of them
     uint32_t part_begin = 0, part_count = 0;   // part_count 0: every chunk
-    // matrices generated on the device (RandomQUBO): PCG64 states staged instead of Q
+    // matrices generated on the device (RandomQUBO, TSP): PCG64 states staged instead of Q
     bool gen = false;
+    bool gen_tsp = false;
     int gen_dtype = 0;
-    int64_t gen_low = 0, gen_high = 0;
+    int64_t gen_low = 0, gen_high = 0;        // RandomQUBO bounds
+    double gen_flow = 0.0, gen_fhigh = 0.0;   // TSP city coordinate bounds
     size_t off_rng = 0;
 };
 }  // namespace qbx
@@ -227,6 +229,18 @@ int stage_matrices(const Layout& lay, char* h, const T* Qpacked, const int64_t* 
     return QBX_OK;
 }
 
+// The generator kernel of a device-drawn batch (matrices into the problems' Q sections).
+hipError_t launch_gen(const Layout& lay, char* dv, hipStream_t stream) {
+    const dim3 grid((unsigned)((lay.count + 63) / 64));
+    if (lay.gen_tsp)
+        hipLaunchKernelGGL(qbx_gen_tsp, grid, dim3(64), 0, stream, (const InstDev*)(dv + lay.off_inst), lay.count,
+                           (const uint64_t*)(dv + lay.off_rng), lay.gen_flow, lay.gen_fhigh, lay.gen_dtype);
+    else
+        hipLaunchKernelGGL(qbx_gen_random, grid, dim3(64), 0, stream, (const InstDev*)(dv + lay.off_inst), lay.count,
+                           (const uint64_t*)(dv + lay.off_rng), lay.gen_low, lay.gen_high, lay.gen_dtype);
+    return hipGetLastError();
+}
+
 // The work of one batch on `stream`: staged copy in, table build, one persistent launch per
 // variant, results out.  Issues no memset and no allocation, so it can be captured in a
 // graph; ev_begin / ev_end (nullable, not used under capture) bracket the kernels.
@@ -235,10 +249,7 @@ int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const c
     HIP_TRY(hipMemcpyAsync(dv, h_stage, lay.stage_bytes, hipMemcpyHostToDevice, stream));
     if (ev_begin) HIP_TRY(hipEventRecord(ev_begin, stream));
     if (lay.gen) {
-        hipLaunchKernelGGL(qbx_gen_random, dim3((unsigned)((lay.count + 63) / 64)), dim3(64), 0, stream,
-                           (const InstDev*)(dv + lay.off_inst), lay.count, (const uint64_t*)(dv + lay.off_rng),
-                           lay.gen_low, lay.gen_high, lay.gen_dtype);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(launch_gen(lay, dv, stream));
     }
     unsigned long long* dticket = (unsigned long long*)(dv + lay.off_ticket);
     hipLaunchKernelGGL(qbx_prep, dim3(lay.count), dim3(256), 0, stream, (InstDev*)(dv + lay.off_inst),
@@ -335,21 +346,40 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     return rc;
 }
 
-// Device-generated RandomQUBO batches: the layout, the staged PCG64 states and (for
-// qbx_random_qubo_read) the matrices copied back.
+// Device-generated batches (RandomQUBO, or TSP when tsp): the layout, the staged PCG64
+// states and (for the *_read calls) the matrices copied back.  For TSP, ns holds the city
+// counts and flow / fhigh the coordinate bounds.
 int gen_prepare(qbx_ctx* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low, int64_t high,
-                int dtype, const qbx_opts_t* opts, Layout& lay) {
+                int dtype, const qbx_opts_t* opts, Layout& lay, bool tsp = false, double flow = 0.0,
+                double fhigh = 0.0) {
     if (!c) return set_err(QBX_EINVAL, "null context");
     if (!rng || !ns) return set_err(QBX_EINVAL, "null argument");
-    if (!gen_valid(low, high, dtype))
+    std::vector<int32_t> sizes;
+    if (tsp) {
+        if (dtype != GEN_FLOAT32 && dtype != GEN_FLOAT64)
+            return set_err(QBX_EINVAL, "TSP cities must be float32 or float64 (dtype %d)", dtype);
+        if (!(flow <= fhigh) || !(fhigh - flow < 1.0e300))
+            return set_err(QBX_EINVAL, "unsupported TSP coordinate bounds [%g, %g)", flow, fhigh);
+        sizes.resize(count > 0 ? (size_t)count : 0u);
+        for (int i = 0; i < count; ++i) {
+            if (ns[i] < 1 || ns[i] > 7)
+                return set_err(QBX_EINVAL, "problem %d: %d cities outside [1, 7]", i, ns[i]);
+            sizes[i] = ns[i] * ns[i];
+        }
+        ns = sizes.data();
+    } else if (!gen_valid(low, high, dtype)) {
         return set_err(QBX_EINVAL, "unsupported generator bounds [%lld, %lld) for dtype %d", (long long)low,
                        (long long)high, dtype);
+    }
     const int kind = (dtype == GEN_INT16 || dtype == GEN_INT32) ? KIND_I32 : KIND_F64;
     int rc = make_layout(c, kind, count, ns, opts, false, lay, true);
     if (rc) return rc;
+    lay.gen_tsp = tsp;
     lay.gen_dtype = dtype;
     lay.gen_low = low;
     lay.gen_high = high;
+    lay.gen_flow = flow;
+    lay.gen_fhigh = fhigh;
     rc = ensure_capacity(c, lay.total_bytes, std::max(lay.total_bytes, sizeof(ResultDev) * (size_t)count));
     if (rc) return rc;
     stage_descriptors(lay, c->hbuf, c->dbuf);
@@ -552,14 +582,9 @@ int qbx_batch_plan_destroy(qbx_batch_plan_t* P) {
     return QBX_OK;
 }
 
-int qbx_batch_random_qubo(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low,
-                          int64_t high, int dtype, const qbx_opts_t* opts, qbx_result_t* out) {
-    if (!out) return set_err(QBX_EINVAL, "null argument");
-    Layout lay;
-    int rc = gen_prepare(c, count, rng, ns, low, high, dtype, opts, lay);
-    if (rc) return rc;
+static int gen_batch(qbx_ctx* c, const Layout& lay, qbx_result_t* out) {
     HIP_TRY(hipSetDevice(c->device));
-    rc = enqueue(c, c->stream, lay, c->dbuf, c->hbuf, c->hbuf, c->ev0, c->ev1);
+    const int rc = enqueue(c, c->stream, lay, c->dbuf, c->hbuf, c->hbuf, c->ev0, c->ev1);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
@@ -567,18 +592,11 @@ int qbx_batch_random_qubo(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const
     return collect(lay, (const ResultDev*)c->hbuf, ms, out);
 }
 
-int qbx_random_qubo_read(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low,
-                         int64_t high, int dtype, void* Qout) {
-    if (!Qout) return set_err(QBX_EINVAL, "null argument");
-    Layout lay;
-    int rc = gen_prepare(c, count, rng, ns, low, high, dtype, nullptr, lay);
-    if (rc) return rc;
+static int gen_read(qbx_ctx* c, const Layout& lay, void* Qout) {
+    const int count = lay.count;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpyAsync(c->dbuf, c->hbuf, lay.stage_bytes, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(qbx_gen_random, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, c->stream,
-                       (const InstDev*)(c->dbuf + lay.off_inst), count, (const uint64_t*)(c->dbuf + lay.off_rng),
-                       low, high, dtype);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_gen(lay, c->dbuf, c->stream));
     // the Q section is contiguous from the first problem's matrix to the last one's
     const size_t q0 = lay.plans[0].off_Q;
     const Plan& pl = lay.plans[count - 1];
@@ -592,6 +610,38 @@ int qbx_random_qubo_read(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const 
         dst += bytes;
     }
     return QBX_OK;
+}
+
+int qbx_batch_random_qubo(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low,
+                          int64_t high, int dtype, const qbx_opts_t* opts, qbx_result_t* out) {
+    if (!out) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    const int rc = gen_prepare(c, count, rng, ns, low, high, dtype, opts, lay);
+    return rc ? rc : gen_batch(c, lay, out);
+}
+
+int qbx_random_qubo_read(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low,
+                         int64_t high, int dtype, void* Qout) {
+    if (!Qout) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    const int rc = gen_prepare(c, count, rng, ns, low, high, dtype, nullptr, lay);
+    return rc ? rc : gen_read(c, lay, Qout);
+}
+
+int qbx_batch_random_tsp(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* n_cities, double low,
+                         double high, int dtype, const qbx_opts_t* opts, qbx_result_t* out) {
+    if (!out) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    const int rc = gen_prepare(c, count, rng, n_cities, 0, 0, dtype, opts, lay, true, low, high);
+    return rc ? rc : gen_batch(c, lay, out);
+}
+
+int qbx_random_tsp_read(qbx_ctx_t* c, int count, const qbx_pcg64_t* rng, const int32_t* n_cities, double low,
+                        double high, int dtype, double* Qout) {
+    if (!Qout) return set_err(QBX_EINVAL, "null argument");
+    Layout lay;
+    const int rc = gen_prepare(c, count, rng, n_cities, 0, 0, dtype, nullptr, lay, true, low, high);
+    return rc ? rc : gen_read(c, lay, Qout);
 }
 
 int qbx_solve_part_i32(qbx_ctx_t* c, const int32_t* Q, int n, const qbx_opts_t* opts, int part, int parts,

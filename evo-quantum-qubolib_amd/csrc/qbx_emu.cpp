@@ -187,6 +187,18 @@ int qbx_emu_random_qubo(const uint64_t* st, int n, int64_t low, int64_t high, in
     return 0;
 }
 
+// TSP QUBO from a PCG64 state, as the device generator builds it: n_cities in 1..7,
+// dtype GEN_FLOAT32 or GEN_FLOAT64 (the cities' type), out: (n_cities^2)^2 doubles.
+int qbx_emu_random_tsp(const uint64_t* st, int nc, double low, double high, int dtype, double* out) {
+    if (nc < 1 || nc > 7 || (dtype != GEN_FLOAT32 && dtype != GEN_FLOAT64)) return -1;
+    Pcg64 g;
+    g.hi = st[0]; g.lo = st[1]; g.inc_hi = st[2]; g.inc_lo = st[3];
+    g.buf32 = 0; g.buf16 = 0; g.has32 = g.has16 = 0;
+    if (dtype == GEN_FLOAT32) gen_tsp<float>(g, nc, low, high, out);
+    else gen_tsp<double>(g, nc, low, high, out);
+    return 0;
+}
+
 // float filter path counters since the last call (buffer overflows, FP64 block re-walks)
 void qbx_emu_take_stats(uint64_t* flushes, uint64_t* rewalks) {
     *flushes = emu_stat_flushes;

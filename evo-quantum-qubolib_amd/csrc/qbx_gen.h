@@ -160,4 +160,82 @@ QBX_HD void gen_random_qubo(Pcg64 g, int n, int64_t low, int64_t high, int dtype
     }
 }
 
+// ---------------------------------------------------------------------------- TSP
+// TSP.init_random_instance(seed, n_cities, low, high, dtype).build_Q_matrix()
+// (src/qubo_lib/problems/tsp.py:50-59, 67-75, 272-321), bit for bit:
+//   * cities: rng.uniform(low, high, (n_cities, 2)) -- per city x then y, each
+//     low + (high - low) * next_double, two roundings -- cast to dtype (C: float or double);
+//   * distances in C, element by element: sqrt((x_i - x_j)^2 + (y_i - y_j)^2), the squares
+//     and the sum rounded separately (no fused multiply-add), the square root correctly
+//     rounded; stored in double;
+//   * Q in double over index a = city * n + time: tour cost w[i][j] at ((i, p), (j, p + 1)),
+//     plus A * C with A = 2 max(w) + 1 and C = -1 on the diagonal, +1 for the same time or
+//     the same city; folded to the upper triangle (Q_ab + Q_ba above the diagonal, one
+//     rounding); cast to dtype and written as double (the solver's element type).
+QBX_HD double gen_next_double(Pcg64& g) { return (double)(pcg64_next(g) >> 11) * 1.1102230246251565e-16; }
+
+QBX_HD float gen_sqrt(float x) { return sqrtf(x); }      // correctly rounded, both builds
+QBX_HD double gen_sqrt(double x) { return sqrt(x); }
+
+template <class C>
+QBX_HD C tsp_distance(C xi, C xj, C yi, C yj) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const C dx = xi - xj;
+    const C dy = yi - yj;
+    const C sx = dx * dx;
+    const C sy = dy * dy;
+    const C s = sx + sy;
+    return gen_sqrt(s);
+}
+
+QBX_HD double tsp_penalty_sum(double q, double A, int c) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const double t = A * (double)c;
+    return q + t;
+}
+
+template <class C>
+QBX_HD void gen_tsp(Pcg64 g, int nc, double low, double high, double* out) {
+    C xs[8], ys[8];
+    const double span = high - low;
+    for (int i = 0; i < nc; ++i) {
+        xs[i] = (C)gen_scale_f64(low, gen_next_double(g), span);
+        ys[i] = (C)gen_scale_f64(low, gen_next_double(g), span);
+    }
+    double w[8][8];
+    double wmax = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        w[i][i] = 0.0;
+        for (int j = i + 1; j < nc; ++j) {
+            const double d = (double)tsp_distance<C>(xs[i], xs[j], ys[i], ys[j]);
+            w[i][j] = d;
+            w[j][i] = d;
+            wmax = d > wmax ? d : wmax;
+        }
+    }
+    const double A = 2.0 * wmax + 1.0;
+    const int N = nc * nc;
+    // full (unfolded) entry (a, b): tour cost plus A times the constraint matrix
+    auto full = [&](int a, int b) -> double {
+        const int ci = a / nc, pa = a - ci * nc, cj = b / nc, pb = b - cj * nc;
+        const double cost = (pb == pa + 1) ? w[ci][cj] : 0.0;
+        int c = 0;
+        if (a == b) c = -1;
+        else if (pa == pb || ci == cj) c = 1;
+        return tsp_penalty_sum(cost, A, c);
+    };
+    for (int a = 0; a < N; ++a) {
+        for (int b = 0; b < N; ++b) {
+            double v = 0.0;
+            if (a == b) v = full(a, a);
+            else if (a < b) v = full(a, b) + full(b, a);
+            out[(size_t)a * N + b] = (double)(C)v;
+        }
+    }
+}
+
 }  // namespace qbx

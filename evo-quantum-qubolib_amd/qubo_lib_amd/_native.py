@@ -162,6 +162,10 @@ _SIGNATURES = {
                                              ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _P(Opts), _P(Result)]),
     "qbx_random_qubo_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
                                             ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]),
+    "qbx_batch_random_tsp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_random_tsp_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, _P(ctypes.c_double)]),
     "qbx_set_walk_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int32),
                                           ctypes.c_int]),
     "qbx_plan_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(Opts),
@@ -434,6 +438,43 @@ class Context:
         _check(self.lib, rc)
         out, o = [], 0
         for n in nsa:
+            out.append(buf[o:o + n * n].reshape(n, n))
+            o += n * n
+        return out
+
+    def _tsp_args(self, seeds, n_cities, dtype):
+        key = np.dtype(dtype).name
+        if key not in ("float32", "float64"):
+            raise TypeError(f"TSP cities of dtype {key} cannot be drawn on the device")
+        states, nca, code = self._gen_args(seeds, n_cities, dtype)
+        return states, nca, code
+
+    def random_tsp_batch(self, seeds, n_cities, low=0, high=100, dtype=np.float32, **kw):
+        """Solve TSP.init_random_instance(seed, n_cities, low, high, dtype) instances whose
+        QUBOs are built on the device (bit for bit the reference's); returns a sequence of
+        Result (as batch())."""
+        states, nca, code = self._tsp_args(seeds, n_cities, dtype)
+        out = (Result * len(seeds))()
+        o = _opts(**kw)
+        with self.lock:
+            rc = self.lib.qbx_batch_random_tsp(self._h, len(seeds), states, nca.ctypes.data_as(_P(ctypes.c_int32)),
+                                               float(low), float(high), code, ctypes.byref(o), out)
+        if rc not in (QBX_OK, QBX_ERANGE):
+            _check(self.lib, rc)
+        return out
+
+    def random_tsp_read(self, seeds, n_cities, low=0, high=100, dtype=np.float32) -> list:
+        """The QUBOs random_tsp_batch would solve, built on the device and copied back
+        (float64, n_cities^2 square)."""
+        states, nca, code = self._tsp_args(seeds, n_cities, dtype)
+        ns = nca.astype(np.int64) ** 2
+        buf = np.empty(int((ns ** 2).sum()), dtype=np.float64)
+        with self.lock:
+            rc = self.lib.qbx_random_tsp_read(self._h, len(seeds), states, nca.ctypes.data_as(_P(ctypes.c_int32)),
+                                              float(low), float(high), code, buf.ctypes.data_as(_P(ctypes.c_double)))
+        _check(self.lib, rc)
+        out, o = [], 0
+        for n in ns:
             out.append(buf[o:o + n * n].reshape(n, n))
             o += n * n
         return out

@@ -19,55 +19,77 @@ from .solution import Solution
 from .state_vectors import StateVectors
 
 
-def _drawable_on_device(problems) -> bool:
-    """Seeded RandomQUBO instances sharing bounds and a dtype the device generator draws."""
+def _device_family(problems):
+    """'random_qubo' for seeded RandomQUBO instances sharing bounds and a dtype the device
+    generator draws, 'tsp' for seeded random TSP instances (1..7 cities) sharing bounds and
+    a float dtype, else None."""
     from ._native import GEN_DTYPES
     from .problems.random_qubo import RandomQUBO
+    from .problems.tsp import TSP
     if not problems:
-        return False
+        return None
     first = problems[0]
+    key = lambda p: (getattr(p, "low", None), getattr(p, "high", None), np.dtype(p.dtype))
+    if all(isinstance(p, TSP) and getattr(p, "seed", None) is not None and getattr(p, "low", None) is not None
+           and 1 <= int(p.n_cities) <= 7 and np.dtype(p.dtype).name in ("float32", "float64")
+           and key(p) == key(first) for p in problems):
+        return "tsp"
     for p in problems:
         if not isinstance(p, RandomQUBO) or getattr(p, "seed", None) is None:
-            return False
+            return None
         if np.dtype(p.dtype).name not in GEN_DTYPES:
-            return False
-        if (p.low, p.high, np.dtype(p.dtype)) != (first.low, first.high, np.dtype(first.dtype)):
-            return False
+            return None
+        if key(p) != key(first):
+            return None
     if np.issubdtype(np.dtype(first.dtype), np.integer):
         # integer draws are solved on the int32 path: its energy bound must hold for any draw
         n = max(int(p.N) for p in problems)
         m = max(abs(int(first.low)), abs(int(first.high) - 1))
         if m * (n + n * (n - 1)) > engine.INT32_BOUND:
-            return False
-    return True
+            return None
+    return "random_qubo"
+
+
+def _drawable_on_device(problems) -> bool:
+    """Whether the device can build these problems' matrices itself (_device_family)."""
+    return _device_family(problems) is not None
 
 
 def solve_batch(problems, device: str = "cuda", B=None, walk_bits=None, inner_bits=None,
                 draw_on_device=None) -> list:
     """Solve every problem exactly; returns one Solution per problem, in order.
 
-    draw_on_device: seeded RandomQUBO batches can have their matrices drawn on the GPU
-    (bit for bit numpy's, random_qubo.py:19-33) instead of by numpy on the host; None
-    selects it for such batches of 64 or more problems."""
+    draw_on_device: seeded RandomQUBO and random TSP batches can have their matrices built
+    on the GPU (bit for bit the reference's, random_qubo.py:19-33, tsp.py:50-75, 272-321)
+    instead of on the host; None selects it for such batches of 64 or more problems."""
     dev = 0
     if isinstance(device, str) and device[:4] == "cuda":
         dev = int(device[5:]) if len(device) > 5 else 0
     else:
         raise ValueError(f"solve_batch runs on the GPU only: {device}")
-    drawable = _drawable_on_device(problems)
-    if draw_on_device and not drawable:
-        raise ValueError("draw_on_device needs seeded RandomQUBO problems with one (low, high, dtype)")
+    family = _device_family(problems)
+    if draw_on_device and family is None:
+        raise ValueError("draw_on_device needs seeded RandomQUBO or random TSP problems with one (low, high, dtype)")
     if draw_on_device is None:
-        draw_on_device = drawable and len(problems) >= 64
+        draw_on_device = family is not None and len(problems) >= 64
     if draw_on_device:
         from . import _native
         ctx = _native.get_context(dev)
         p0 = problems[0]
-        seeds, ns = [int(p.seed) for p in problems], [int(p.N) for p in problems]
-        res = ctx.random_qubo_batch(seeds, ns, p0.low, p0.high, p0.dtype, tie_B=B, walk_bits=walk_bits,
-                                    inner_bits=inner_bits)
-        mats = ctx.random_qubo_read(seeds, ns, p0.low, p0.high, p0.dtype)   # for the Ising offsets
-        raws = [engine.raw_from_native(r, integer=np.issubdtype(np.dtype(p0.dtype), np.integer)) for r in res]
+        seeds = [int(p.seed) for p in problems]
+        kw = dict(tie_B=B, walk_bits=walk_bits, inner_bits=inner_bits)
+        if family == "tsp":
+            ncs = [int(p.n_cities) for p in problems]
+            res = ctx.random_tsp_batch(seeds, ncs, p0.low, p0.high, p0.dtype, **kw)
+            mats = ctx.random_tsp_read(seeds, ncs, p0.low, p0.high, p0.dtype)   # for the Ising offsets
+            integer = False
+        else:
+            ns = [int(p.N) for p in problems]
+            res = ctx.random_qubo_batch(seeds, ns, p0.low, p0.high, p0.dtype, **kw)
+            mats = ctx.random_qubo_read(seeds, ns, p0.low, p0.high, p0.dtype)   # for the Ising offsets
+            integer = np.issubdtype(np.dtype(p0.dtype), np.integer)
+        raws = [engine.raw_from_native(r, integer=integer) for r in res]
+        mats = [m.astype(p0.dtype) for m in mats]    # the dtype build_Q_matrix returns
     else:
         mats = [np.asarray(p.build_Q_matrix(use_gpu=False)) for p in problems]
         raws = engine.solve_matrices(mats, device=dev, tie_B=B, walk_bits=walk_bits, inner_bits=inner_bits)

@@ -78,6 +78,7 @@ class TSP(ProblemABC):
         cities = rng.uniform(low=low, high=high, size=(n_cities, 2)).astype(dtype)
         obj = TSP.init_from_cities(cities, dtype)
         obj.seed = seed
+        obj.low, obj.high = low, high      # what a device-side build needs besides the seed
         return obj
 
     def build_Q_matrix(self, use_gpu: bool = False) -> np.ndarray:

@@ -168,6 +168,17 @@ int qbx_batch_random_qubo(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, con
                           int64_t low, int64_t high, int dtype, const qbx_opts_t *opts, qbx_result_t *out);
 int qbx_random_qubo_read(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *ns,
                          int64_t low, int64_t high, int dtype, void *Qout);
+/* Batches of TSP.init_random_instance(seed, n_cities, low, high, dtype) instances (1..7
+ * cities, dtype QBX_GEN_FLOAT32 or QBX_GEN_FLOAT64 for the city coordinates) whose QUBOs
+ * are built on the device, bit for bit as the reference builds them (tsp.py:50-75,
+ * 272-321: uniform cities, distances in the cities' type, tour cost plus A times the
+ * constraints, folded to the upper triangle, cast to dtype); solved on the FP64 path.
+ * The reference builds them with Python loops on the host.  qbx_random_tsp_read returns
+ * the matrices ((n_cities^2)^2 doubles each, packed) instead of solving them. */
+int qbx_batch_random_tsp(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *n_cities,
+                         double low, double high, int dtype, const qbx_opts_t *opts, qbx_result_t *out);
+int qbx_random_tsp_read(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *n_cities,
+                        double low, double high, int dtype, double *Qout);
 
 /* ------------------------------------------------------------------- tiling
  * Walk bits (L: states per lane = 2^L, lanes per problem = 2^(n-L)) used when

@@ -44,6 +44,28 @@ def test_host_build_of_generator_matches_numpy(emu, seed, n, low, high, dtype):
     assert np.array_equal(out, ref.astype(out.dtype))
 
 
+TSP_CASES = [  # (seed, n_cities, low, high, dtype)
+    (42, 5, 0, 100, np.float32), (42, 6, 0, 100, np.float32), (1, 7, 0, 100, np.float32),
+    (3, 4, -50, 50, np.float32), (9, 2, 0, 1, np.float32), (5, 3, 0, 1000, np.float64),
+    (8, 6, 0, 100, np.float64), (11, 1, 0, 100, np.float32),
+]
+
+
+def _tsp_ref(seed, nc, low, high, dtype):
+    from qubo_lib_amd.problems import TSP
+    return TSP.init_random_instance(seed=seed, n_cities=nc, low=low, high=high, dtype=dtype).build_Q_matrix()
+
+
+@pytest.mark.parametrize("seed,nc,low,high,dtype", TSP_CASES)
+def test_host_build_of_tsp_generator_matches_numpy(emu, seed, nc, low, high, dtype):
+    emu.qbx_emu_random_tsp.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    ref = _tsp_ref(seed, nc, low, high, dtype)
+    out = np.empty((nc * nc, nc * nc), dtype=np.float64)
+    assert emu.qbx_emu_random_tsp(_state(seed), nc, low, high, CODES[dtype], out.ctypes.data) == 0
+    assert np.array_equal(out, ref.astype(np.float64))
+
+
 @pytest.mark.gpu
 def test_device_generator_matches_numpy(gpu_ctx):
     for dtype in (np.int16, np.int32, np.float32, np.float64):
@@ -76,6 +98,43 @@ def test_solve_batch_draws_random_qubo_on_device(gpu_ctx):
     from qubo_lib_amd.batch import solve_batch
     probs = [RandomQUBO.init_random_instance(seed=100 + i, N=10 + i % 14, dtype=np.int16) for i in range(80)]
     dev = solve_batch(probs)                       # 80 seeded RandomQUBO: drawn on the device
+    host = solve_batch(probs, draw_on_device=False)
+    for a, b in zip(dev, host):
+        assert a.best_E == b.best_E and list(a.best_X) == list(b.best_X)
+        assert a.ising_offset == b.ising_offset
+
+
+@pytest.mark.gpu
+def test_device_tsp_matrices_match_reference_build(gpu_ctx):
+    for dtype in (np.float32, np.float64):
+        sel = [c for c in TSP_CASES if c[4] is dtype]
+        for low, high in sorted({(c[2], c[3]) for c in sel}):
+            group = [c for c in sel if (c[2], c[3]) == (low, high)]
+            mats = gpu_ctx.random_tsp_read([c[0] for c in group], [c[1] for c in group], low, high, dtype)
+            for (seed, nc, _, _, _), m in zip(group, mats):
+                ref = _tsp_ref(seed, nc, low, high, dtype)
+                assert np.array_equal(m, ref.astype(np.float64)), (seed, nc, low, high, dtype)
+
+
+@pytest.mark.gpu
+def test_device_built_tsp_batch_equals_host_built_solves(gpu_ctx):
+    from qubo_lib_amd import engine
+    rng = np.random.default_rng(18)
+    seeds = [int(s) for s in rng.integers(0, 2 ** 31, 40)]
+    ncs = [int(c) for c in rng.integers(2, 6, 40)]
+    for dtype in (np.float32, np.float64):
+        res = gpu_ctx.random_tsp_batch(seeds, ncs, 0, 100, dtype)
+        for seed, nc, r in zip(seeds, ncs, res):
+            alone = engine.solve_matrix(_tsp_ref(seed, nc, 0, 100, dtype))
+            assert r.status == 0 and r.state == alone.state and r.E == alone.E, (seed, nc, dtype)
+
+
+@pytest.mark.gpu
+def test_solve_batch_builds_tsp_on_device(gpu_ctx):
+    from qubo_lib_amd.batch import solve_batch
+    from qubo_lib_amd.problems import TSP
+    probs = [TSP.init_random_instance(seed=200 + i, n_cities=2 + i % 4) for i in range(72)]
+    dev = solve_batch(probs)                       # 72 seeded random TSP: built on the device
     host = solve_batch(probs, draw_on_device=False)
     for a, b in zip(dev, host):
         assert a.best_E == b.best_E and list(a.best_X) == list(b.best_X)
