@@ -293,6 +293,7 @@ def batch(args, ctx, devinfo):
                for a, b, e in zip(res, plain, res_e))
     states = float(sum(2.0 ** q.shape[0] for q in Qs))
     drawn = device_drawn_random_batch(args, ctx)
+    drawn_tsp = device_built_tsp_batch(args, ctx)
     return {
         "metric": "batched solves/s (4096-instance uneven mix)",
         "value": len(Qs) / wall_b,
@@ -311,6 +312,7 @@ def batch(args, ctx, devinfo):
         "speedup_vs_plain": wall_p / wall_b,
         "identical_answers": same,
         "random_qubo_drawn_on_device": drawn,
+        "tsp_built_on_device": drawn_tsp,
         "states_per_s": states / wall_b,
     }
 
@@ -336,6 +338,31 @@ def device_drawn_random_batch(args, ctx, count: int = 4096) -> dict:
             "host_drawn": {"solves_per_s": count / (t2 - t0), "numpy_ms": (t1 - t0) * 1e3,
                            "solve_ms": (t2 - t1) * 1e3},
             "device_drawn": {"solves_per_s": count / (t3 - t2), "ms": (t3 - t2) * 1e3,
+                             "kernel_ms": dev[0].kernel_ms},
+            "identical_answers": same}
+
+
+def device_built_tsp_batch(args, ctx, count: int = 4096) -> dict:
+    """TSP.init_random_instance(seed, 4 or 5 cities, [0, 100), float32) x count, end to end:
+    cities drawn and QUBOs built on the host and staged vs both done on the device (same
+    bits).  Timed from the seeds: the host leg includes drawing the cities."""
+    from qubo_lib_amd.problems import TSP
+    rng = np.random.default_rng(args.seed + 1)
+    seeds = [int(s) for s in rng.integers(0, 2 ** 31, count)]
+    ncs = [int(c) for c in rng.integers(4, 6, count)]
+    ctx.random_tsp_batch(seeds[:64], ncs[:64], 0, 100, np.float32)          # warm-up
+    t0 = time.perf_counter()
+    mats = [TSP.init_random_instance(seed=s, n_cities=c).build_Q_matrix() for s, c in zip(seeds, ncs)]
+    t1 = time.perf_counter()
+    host = ctx.batch([m.astype(np.float64) for m in mats], kind="f64")
+    t2 = time.perf_counter()
+    dev = ctx.random_tsp_batch(seeds, ncs, 0, 100, np.float32)
+    t3 = time.perf_counter()
+    same = all((a.E, a.state) == (b.E, b.state) for a, b in zip(host, dev))
+    return {"workload": f"{count} TSP float32 cities in [0,100)^2, 4 or 5 cities (N = 16 or 25)",
+            "host_built": {"solves_per_s": count / (t2 - t0), "build_ms": (t1 - t0) * 1e3,
+                           "solve_ms": (t2 - t1) * 1e3},
+            "device_built": {"solves_per_s": count / (t3 - t2), "ms": (t3 - t2) * 1e3,
                              "kernel_ms": dev[0].kernel_ms},
             "identical_answers": same}
 

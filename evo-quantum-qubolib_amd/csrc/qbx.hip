@@ -58,12 +58,12 @@ struct Layout {
     size_t total_bytes = 0;
     // one part of a single problem's chunks (rank-space shard), or all of them
     uint32_t part_begin = 0, part_count = 0;   // part_count 0: every chunk
-    // matrices generated on the device (RandomQUBO, TSP): PCG64 states staged instead of Q
+    // matrices generated on the device: a small record per problem is staged instead of Q
+    // (RandomQUBO: the PCG64 state; TSP: the city distances)
     bool gen = false;
     bool gen_tsp = false;
     int gen_dtype = 0;
     int64_t gen_low = 0, gen_high = 0;        // RandomQUBO bounds
-    double gen_flow = 0.0, gen_fhigh = 0.0;   // TSP city coordinate bounds
     size_t off_rng = 0;
 };
 }  // namespace qbx
@@ -106,7 +106,7 @@ int ensure_capacity(qbx_ctx* c, size_t dbytes, size_t hbytes) {
 }
 
 int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_opts_t* opts, bool single,
-                Layout& lay, bool gen = false) {
+                Layout& lay, size_t gen_rec = 0) {
     if (count <= 0) return set_err(QBX_EINVAL, "count must be positive, got %d", count);
     if (!ns) return set_err(QBX_EINVAL, "null argument");
     qbx_opts_t o = {-1, -1, -1, 0};
@@ -122,9 +122,9 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     off = align256(off + sizeof(InstDev) * count);
     lay.off_ranges = off;
     off = align256(off + sizeof(TileRange) * count);
-    lay.gen = gen;
+    lay.gen = gen_rec > 0;
     lay.off_rng = off;
-    if (gen) off = align256(off + 4 * sizeof(uint64_t) * count);
+    if (lay.gen) off = align256(off + gen_rec * count);
     const size_t q_begin = off;
     const PlanMode mode = single ? PLAN_SINGLE : PLAN_BATCH;
     for (int i = 0; i < count; ++i) {
@@ -134,9 +134,9 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
         p.off_Q = off;
         off = align256(off + lay.esize * (size_t)p.n * p.n);
     }
-    // staged from the host: [instances | ranges | Q data], or [instances | ranges | PCG64
-    // states] when the matrices are generated on the device
-    lay.stage_bytes = gen ? q_begin : off;
+    // staged from the host: [instances | ranges | Q data], or [instances | ranges | generator
+    // records] when the matrices are generated on the device
+    lay.stage_bytes = lay.gen ? q_begin : off;
     for (int i = 0; i < count; ++i) {
         Plan& p = plans[i];
         p.off_W = off;
@@ -234,7 +234,7 @@ hipError_t launch_gen(const Layout& lay, char* dv, hipStream_t stream) {
     const dim3 grid((unsigned)((lay.count + 63) / 64));
     if (lay.gen_tsp)
         hipLaunchKernelGGL(qbx_gen_tsp, grid, dim3(64), 0, stream, (const InstDev*)(dv + lay.off_inst), lay.count,
-                           (const uint64_t*)(dv + lay.off_rng), lay.gen_flow, lay.gen_fhigh, lay.gen_dtype);
+                           (const double*)(dv + lay.off_rng), lay.gen_dtype);
     else
         hipLaunchKernelGGL(qbx_gen_random, grid, dim3(64), 0, stream, (const InstDev*)(dv + lay.off_inst), lay.count,
                            (const uint64_t*)(dv + lay.off_rng), lay.gen_low, lay.gen_high, lay.gen_dtype);
@@ -346,14 +346,16 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     return rc;
 }
 
-// Device-generated batches (RandomQUBO, or TSP when tsp): the layout, the staged PCG64
-// states and (for the *_read calls) the matrices copied back.  For TSP, ns holds the city
-// counts and flow / fhigh the coordinate bounds.
+// Device-generated batches (RandomQUBO, or TSP when tsp): the layout and the staged
+// records, PCG64 states for RandomQUBO, for TSP the city distances drawn and measured here
+// with the C library's pow (tsp_distances).  For TSP, ns holds the city counts and flow /
+// fhigh the coordinate bounds.
 int gen_prepare(qbx_ctx* c, int count, const qbx_pcg64_t* rng, const int32_t* ns, int64_t low, int64_t high,
                 int dtype, const qbx_opts_t* opts, Layout& lay, bool tsp = false, double flow = 0.0,
                 double fhigh = 0.0) {
     if (!c) return set_err(QBX_EINVAL, "null context");
     if (!rng || !ns) return set_err(QBX_EINVAL, "null argument");
+    const int32_t* n_cities = ns;
     std::vector<int32_t> sizes;
     if (tsp) {
         if (dtype != GEN_FLOAT32 && dtype != GEN_FLOAT64)
@@ -362,8 +364,8 @@ int gen_prepare(qbx_ctx* c, int count, const qbx_pcg64_t* rng, const int32_t* ns
             return set_err(QBX_EINVAL, "unsupported TSP coordinate bounds [%g, %g)", flow, fhigh);
         sizes.resize(count > 0 ? (size_t)count : 0u);
         for (int i = 0; i < count; ++i) {
-            if (ns[i] < 1 || ns[i] > 7)
-                return set_err(QBX_EINVAL, "problem %d: %d cities outside [1, 7]", i, ns[i]);
+            if (ns[i] < 1 || ns[i] > TSP_MAX_CITIES)
+                return set_err(QBX_EINVAL, "problem %d: %d cities outside [1, %d]", i, ns[i], TSP_MAX_CITIES);
             sizes[i] = ns[i] * ns[i];
         }
         ns = sizes.data();
@@ -372,17 +374,33 @@ int gen_prepare(qbx_ctx* c, int count, const qbx_pcg64_t* rng, const int32_t* ns
                        (long long)high, dtype);
     }
     const int kind = (dtype == GEN_INT16 || dtype == GEN_INT32) ? KIND_I32 : KIND_F64;
-    int rc = make_layout(c, kind, count, ns, opts, false, lay, true);
+    const size_t rec = tsp ? TSP_MAX_PAIRS * sizeof(double) : 4 * sizeof(uint64_t);
+    int rc = make_layout(c, kind, count, ns, opts, false, lay, rec);
     if (rc) return rc;
     lay.gen_tsp = tsp;
     lay.gen_dtype = dtype;
     lay.gen_low = low;
     lay.gen_high = high;
-    lay.gen_flow = flow;
-    lay.gen_fhigh = fhigh;
     rc = ensure_capacity(c, lay.total_bytes, std::max(lay.total_bytes, sizeof(ResultDev) * (size_t)count));
     if (rc) return rc;
     stage_descriptors(lay, c->hbuf, c->dbuf);
+    if (tsp) {
+        double* hd = (double*)(c->hbuf + lay.off_rng);
+        for (int i = 0; i < count; ++i) {
+            Pcg64 g;
+            g.hi = rng[i].state_hi;
+            g.lo = rng[i].state_lo;
+            g.inc_hi = rng[i].inc_hi;
+            g.inc_lo = rng[i].inc_lo;
+            g.buf32 = 0;
+            g.buf16 = 0;
+            g.has32 = g.has16 = 0;
+            double* d = hd + (size_t)i * TSP_MAX_PAIRS;
+            if (dtype == GEN_FLOAT32) tsp_distances<float>(g, n_cities[i], flow, fhigh, d);
+            else tsp_distances<double>(g, n_cities[i], flow, fhigh, d);
+        }
+        return QBX_OK;
+    }
     uint64_t* hr = (uint64_t*)(c->hbuf + lay.off_rng);
     for (int i = 0; i < count; ++i) {
         hr[4 * i + 0] = rng[i].state_hi;
@@ -608,6 +626,20 @@ static int gen_read(qbx_ctx* c, const Layout& lay, void* Qout) {
         const size_t bytes = lay.esize * (size_t)lay.plans[i].n * lay.plans[i].n;
         memcpy(dst, c->hbuf + lay.plans[i].off_Q, bytes);
         dst += bytes;
+    }
+    return QBX_OK;
+}
+
+int qbx_pcg64_from_seeds(int count, const uint64_t* seeds, qbx_pcg64_t* out) {
+    if (count < 0) return set_err(QBX_EINVAL, "count must be non-negative, got %d", count);
+    if (count > 0 && (!seeds || !out)) return set_err(QBX_EINVAL, "null argument");
+    for (int i = 0; i < count; ++i) {
+        Pcg64 g;
+        pcg64_from_seed(seeds[i], g);
+        out[i].state_hi = g.hi;
+        out[i].state_lo = g.lo;
+        out[i].inc_hi = g.inc_hi;
+        out[i].inc_lo = g.inc_lo;
     }
     return QBX_OK;
 }

@@ -234,26 +234,19 @@ static __global__ __launch_bounds__(64) void qbx_gen_random(const InstDev* insts
     else gen_random_qubo<double>(g, I.n, low, high, dtype, (double*)I.Q);
 }
 
-// One thread per problem: the TSP QUBO of n_cities = sqrt(n) cities drawn from the
-// problem's PCG64 state (tsp.py:50-75, 272-321), as double, into I.Q.  dtype: the cities'
-// type (GEN_FLOAT32 | GEN_FLOAT64).
-static __global__ __launch_bounds__(64) void qbx_gen_tsp(const InstDev* insts, int count, const uint64_t* rng,
-                                                        double low, double high, int dtype) {
+// One thread per problem: the TSP QUBO of n_cities = sqrt(n) cities from their distances
+// (TSP_MAX_PAIRS doubles per problem, the upper triangle row by row; tsp.py:285-321), as
+// double, into I.Q.  dtype: the cities' type (GEN_FLOAT32 | GEN_FLOAT64).
+static __global__ __launch_bounds__(64) void qbx_gen_tsp(const InstDev* insts, int count, const double* dist,
+                                                        int dtype) {
     const int i = (int)(blockIdx.x * 64u + threadIdx.x);
     if (i >= count) return;
     const InstDev& I = insts[i];
     int nc = 1;
     while ((nc + 1) * (nc + 1) <= I.n) ++nc;
-    Pcg64 g;
-    g.hi = rng[4 * i + 0];
-    g.lo = rng[4 * i + 1];
-    g.inc_hi = rng[4 * i + 2];
-    g.inc_lo = rng[4 * i + 3];
-    g.buf32 = 0;
-    g.buf16 = 0;
-    g.has32 = g.has16 = 0;
-    if (dtype == GEN_FLOAT32) gen_tsp<float>(g, nc, low, high, (double*)I.Q);
-    else gen_tsp<double>(g, nc, low, high, (double*)I.Q);
+    const double* d = dist + (size_t)i * TSP_MAX_PAIRS;
+    if (dtype == GEN_FLOAT32) tsp_qubo<float>(nc, d, (double*)I.Q);
+    else tsp_qubo<double>(nc, d, (double*)I.Q);
 }
 
 // ------------------------------------------------------------- tile commit / finalize

@@ -190,12 +190,18 @@ int qbx_emu_random_qubo(const uint64_t* st, int n, int64_t low, int64_t high, in
 // TSP QUBO from a PCG64 state, as the device generator builds it: n_cities in 1..7,
 // dtype GEN_FLOAT32 or GEN_FLOAT64 (the cities' type), out: (n_cities^2)^2 doubles.
 int qbx_emu_random_tsp(const uint64_t* st, int nc, double low, double high, int dtype, double* out) {
-    if (nc < 1 || nc > 7 || (dtype != GEN_FLOAT32 && dtype != GEN_FLOAT64)) return -1;
+    if (nc < 1 || nc > TSP_MAX_CITIES || (dtype != GEN_FLOAT32 && dtype != GEN_FLOAT64)) return -1;
     Pcg64 g;
     g.hi = st[0]; g.lo = st[1]; g.inc_hi = st[2]; g.inc_lo = st[3];
     g.buf32 = 0; g.buf16 = 0; g.has32 = g.has16 = 0;
-    if (dtype == GEN_FLOAT32) gen_tsp<float>(g, nc, low, high, out);
-    else gen_tsp<double>(g, nc, low, high, out);
+    double d[TSP_MAX_PAIRS];
+    if (dtype == GEN_FLOAT32) {
+        tsp_distances<float>(g, nc, low, high, d);
+        tsp_qubo<float>(nc, d, out);
+    } else {
+        tsp_distances<double>(g, nc, low, high, d);
+        tsp_qubo<double>(nc, d, out);
+    }
     return 0;
 }
 

@@ -15,6 +15,7 @@
 // multiply and one rounded add (not a fused multiply-add), as numpy evaluates it.  All of this was
 // checked against numpy 2.2's Generator on the CPU (tests/test_generators_device.py).
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #include "qbx_walk.h"
@@ -71,6 +72,64 @@ QBX_HD uint16_t pcg64_u16(Pcg64& g) {
     g.buf16 = v >> 16;
     g.has16 = 1;
     return (uint16_t)v;
+}
+
+// numpy's Generator(PCG64(seed)) for an integer seed 0 <= seed < 2^64, on the host: the
+// SeedSequence hash of the seed's 32-bit words (numpy/random/bit_generator.pyx: pool of
+// four words, hashmix / mix with its published constants, generate_state(4, uint64) as
+// eight 32-bit words, low word first), then PCG64's seeding (pcg64.h pcg64_set_seed:
+// state = 0, inc = (seq << 1) | 1, step, state += initstate, step) with
+// initstate = val[0]:val[1] and seq = val[2]:val[3].  Pinned against numpy on many seeds.
+inline uint32_t seedseq_hashmix(uint32_t v, uint32_t& h) {
+    v ^= h;
+    h *= 0x931e8875u;
+    v *= h;
+    return v ^ (v >> 16);
+}
+inline uint32_t seedseq_mix(uint32_t x, uint32_t y) {
+    const uint32_t r = 0xca01f9ddu * x - 0x4973f715u * y;
+    return r ^ (r >> 16);
+}
+inline void pcg64_step(Pcg64& g) {
+    const uint64_t Mhi = 0x2360ED051FC65DA4ull, Mlo = 0x4385DF649FCCF645ull;
+    const unsigned __int128 s = ((unsigned __int128)g.hi << 64) | g.lo;
+    const unsigned __int128 inc = ((unsigned __int128)g.inc_hi << 64) | g.inc_lo;
+    const unsigned __int128 t = s * (((unsigned __int128)Mhi << 64) | Mlo) + inc;
+    g.hi = (uint64_t)(t >> 64);
+    g.lo = (uint64_t)t;
+}
+inline void pcg64_from_seed(uint64_t seed, Pcg64& g) {
+    uint32_t ent[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const int n_ent = (seed >> 32) ? 2 : 1;
+    uint32_t pool[4];
+    uint32_t h = 0x43b0d7e5u;
+    for (int i = 0; i < 4; ++i) pool[i] = seedseq_hashmix(i < n_ent ? ent[i] : 0u, h);
+    for (int src = 0; src < 4; ++src)
+        for (int dst = 0; dst < 4; ++dst)
+            if (src != dst) pool[dst] = seedseq_mix(pool[dst], seedseq_hashmix(pool[src], h));
+    uint32_t w[8];
+    uint32_t hb = 0x8b51f9ddu;
+    for (int i = 0; i < 8; ++i) {
+        uint32_t v = pool[i & 3] ^ hb;
+        hb *= 0x58f38dedu;
+        v *= hb;
+        w[i] = v ^ (v >> 16);
+    }
+    uint64_t val[4];
+    for (int k = 0; k < 4; ++k) val[k] = (uint64_t)w[2 * k] | ((uint64_t)w[2 * k + 1] << 32);
+    g.hi = g.lo = 0;
+    const unsigned __int128 seq = ((unsigned __int128)val[2] << 64) | val[3];
+    const unsigned __int128 inc = (seq << 1) | 1u;
+    g.inc_hi = (uint64_t)(inc >> 64);
+    g.inc_lo = (uint64_t)inc;
+    pcg64_step(g);
+    const unsigned __int128 s = (((unsigned __int128)g.hi << 64) | g.lo) + (((unsigned __int128)val[0] << 64) | val[1]);
+    g.hi = (uint64_t)(s >> 64);
+    g.lo = (uint64_t)s;
+    pcg64_step(g);
+    g.buf32 = 0;
+    g.buf16 = 0;
+    g.has32 = g.has16 = 0;
 }
 
 // value in [0, rng] (rng = high - low - 1 >= 1)
@@ -162,32 +221,55 @@ QBX_HD void gen_random_qubo(Pcg64 g, int n, int64_t low, int64_t high, int dtype
 
 // ---------------------------------------------------------------------------- TSP
 // TSP.init_random_instance(seed, n_cities, low, high, dtype).build_Q_matrix()
-// (src/qubo_lib/problems/tsp.py:50-59, 67-75, 272-321), bit for bit:
+// (src/qubo_lib/problems/tsp.py:50-59, 67-75, 272-321), bit for bit, in two parts.
+//
+// tsp_distances (host): the cities and their distances.
 //   * cities: rng.uniform(low, high, (n_cities, 2)) -- per city x then y, each
 //     low + (high - low) * next_double, two roundings -- cast to dtype (C: float or double);
-//   * distances in C, element by element: sqrt((x_i - x_j)^2 + (y_i - y_j)^2), the squares
-//     and the sum rounded separately (no fused multiply-add), the square root correctly
-//     rounded; stored in double;
-//   * Q in double over index a = city * n + time: tour cost w[i][j] at ((i, p), (j, p + 1)),
-//     plus A * C with A = 2 max(w) + 1 and C = -1 on the diagonal, +1 for the same time or
-//     the same city; folded to the upper triangle (Q_ab + Q_ba above the diagonal, one
-//     rounding); cast to dtype and written as double (the solver's element type).
+//   * distances in C, element by element as tsp.py:279 evaluates them on numpy scalars:
+//     sqrt((x_i - x_j) ** 2 + (y_i - y_j) ** 2).  numpy's scalar `**` is the C library's
+//     powf / pow, and pow(x, 2) is not always the correctly rounded x * x (glibc's powf and
+//     pow are accurate to < 1 ulp, not correctly rounded): about one square in 1500 of
+//     float32 coordinates in [0, 100) differs in the last bit.  So the squares are taken
+//     with the C library's own pow, on the host, with the exponent read through a volatile
+//     so that the compiler cannot turn pow(x, 2) into x * x; the sum is one rounding and
+//     sqrtf / sqrt are correctly rounded (IEEE 754), as numpy's np.sqrt.
+//   Stored as double, the upper triangle row by row (i < j), TSP_MAX_PAIRS per problem.
+//
+// tsp_qubo (host and device): Q in double over index a = city * n + time: tour cost
+//   w[i][j] at ((i, p), (j, p + 1)), plus A * C with A = 2 max(w) + 1 and C = -1 on the
+//   diagonal, +1 for the same time or the same city; folded to the upper triangle
+//   (Q_ab + Q_ba above the diagonal, one rounding); cast to dtype and written as double
+//   (the solver's element type).  The multiply and the add of A * C + cost are rounded
+//   separately (no fused multiply-add), as numpy does them.
+constexpr int TSP_MAX_CITIES = 7;
+constexpr int TSP_MAX_PAIRS = TSP_MAX_CITIES * (TSP_MAX_CITIES - 1) / 2;
+
 QBX_HD double gen_next_double(Pcg64& g) { return (double)(pcg64_next(g) >> 11) * 1.1102230246251565e-16; }
 
-QBX_HD float gen_sqrt(float x) { return sqrtf(x); }      // correctly rounded, both builds
-QBX_HD double gen_sqrt(double x) { return sqrt(x); }
+// host only (plain functions: the device pass parses them and emits nothing)
+inline volatile float tsp_two_f = 2.0f;     // exponents the compiler cannot see: pow stays a call
+inline volatile double tsp_two_d = 2.0;
+inline float tsp_square(float x) { return powf(x, tsp_two_f); }
+inline double tsp_square(double x) { return pow(x, tsp_two_d); }
+inline float tsp_sqrt(float x) { return sqrtf(x); }
+inline double tsp_sqrt(double x) { return sqrt(x); }
 
 template <class C>
-QBX_HD C tsp_distance(C xi, C xj, C yi, C yj) {
-#if defined(__clang__)
-#pragma clang fp contract(off)
-#endif
-    const C dx = xi - xj;
-    const C dy = yi - yj;
-    const C sx = dx * dx;
-    const C sy = dy * dy;
-    const C s = sx + sy;
-    return gen_sqrt(s);
+inline void tsp_distances(Pcg64 g, int nc, double low, double high, double* d) {
+    C xs[TSP_MAX_CITIES], ys[TSP_MAX_CITIES];
+    const double span = high - low;
+    for (int i = 0; i < nc; ++i) {
+        xs[i] = (C)gen_scale_f64(low, gen_next_double(g), span);
+        ys[i] = (C)gen_scale_f64(low, gen_next_double(g), span);
+    }
+    int k = 0;
+    for (int i = 0; i < nc; ++i)
+        for (int j = i + 1; j < nc; ++j) {
+            const C sx = tsp_square((C)(xs[i] - xs[j]));
+            const C sy = tsp_square((C)(ys[i] - ys[j]));
+            d[k++] = (double)tsp_sqrt((C)(sx + sy));
+        }
 }
 
 QBX_HD double tsp_penalty_sum(double q, double A, int c) {
@@ -199,22 +281,16 @@ QBX_HD double tsp_penalty_sum(double q, double A, int c) {
 }
 
 template <class C>
-QBX_HD void gen_tsp(Pcg64 g, int nc, double low, double high, double* out) {
-    C xs[8], ys[8];
-    const double span = high - low;
-    for (int i = 0; i < nc; ++i) {
-        xs[i] = (C)gen_scale_f64(low, gen_next_double(g), span);
-        ys[i] = (C)gen_scale_f64(low, gen_next_double(g), span);
-    }
-    double w[8][8];
+QBX_HD void tsp_qubo(int nc, const double* d, double* out) {
+    double w[TSP_MAX_CITIES][TSP_MAX_CITIES];
     double wmax = 0.0;
+    int k = 0;
     for (int i = 0; i < nc; ++i) {
         w[i][i] = 0.0;
         for (int j = i + 1; j < nc; ++j) {
-            const double d = (double)tsp_distance<C>(xs[i], xs[j], ys[i], ys[j]);
-            w[i][j] = d;
-            w[j][i] = d;
-            wmax = d > wmax ? d : wmax;
+            w[i][j] = w[j][i] = d[k];
+            wmax = d[k] > wmax ? d[k] : wmax;
+            ++k;
         }
     }
     const double A = 2.0 * wmax + 1.0;

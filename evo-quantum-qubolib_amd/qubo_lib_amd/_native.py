@@ -14,7 +14,7 @@ import numpy as np
 
 __all__ = [
     "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
-    "ServerStats", "Pcg64State", "GEN_DTYPES", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
+    "ServerStats", "Pcg64State", "pcg64_states", "GEN_DTYPES", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
     "tuning_path", "PATHS", "PAIR_BOUND",
 ]
 
@@ -98,6 +98,23 @@ class Pcg64State(ctypes.Structure):
     ]
 
 
+def pcg64_states(seeds, lib=None):
+    """The states numpy's Generator(PCG64(seed)) starts from: computed by the library
+    (qbx_pcg64_from_seeds) for integer seeds in [0, 2^64), else by numpy's SeedSequence."""
+    states = (Pcg64State * len(seeds))()
+    if all(isinstance(s, (int, np.integer)) and 0 <= int(s) < 2 ** 64 for s in seeds):
+        lib = lib or load_library()
+        arr = np.ascontiguousarray(np.asarray([int(s) for s in seeds], dtype=np.uint64))
+        _check(lib, lib.qbx_pcg64_from_seeds(len(seeds), arr.ctypes.data_as(_P(ctypes.c_uint64)), states))
+        return states
+    from numpy.random import PCG64
+    for st, seed in zip(states, seeds):
+        s = PCG64(seed).state["state"]
+        st.state_hi, st.state_lo = s["state"] >> 64, s["state"] & ((1 << 64) - 1)
+        st.inc_hi, st.inc_lo = s["inc"] >> 64, s["inc"] & ((1 << 64) - 1)
+    return states
+
+
 GEN_DTYPES = {"int16": 16, "int32": 32, "float32": -32, "float64": -64}
 
 
@@ -158,6 +175,7 @@ _SIGNATURES = {
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
     "qbx_batch_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64),
                                      _P(ctypes.c_int32), _P(Opts), _P(Result)]),
+    "qbx_pcg64_from_seeds": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_uint64), _P(Pcg64State)]),
     "qbx_batch_random_qubo": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
                                              ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _P(Opts), _P(Result)]),
     "qbx_random_qubo_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(Pcg64State), _P(ctypes.c_int32),
@@ -397,14 +415,8 @@ class Context:
         return out
 
     # -- RandomQUBO batches drawn on the device ----------------------------------------
-    @staticmethod
-    def _gen_args(seeds, ns, dtype):
-        from numpy.random import PCG64
-        states = (Pcg64State * len(seeds))()
-        for st, seed in zip(states, seeds):
-            s = PCG64(seed).state["state"]
-            st.state_hi, st.state_lo = s["state"] >> 64, s["state"] & ((1 << 64) - 1)
-            st.inc_hi, st.inc_lo = s["inc"] >> 64, s["inc"] & ((1 << 64) - 1)
+    def _gen_args(self, seeds, ns, dtype):
+        states = pcg64_states(seeds, self.lib)
         nsa = np.ascontiguousarray(np.asarray(ns, dtype=np.int32))
         if nsa.size != len(seeds):
             raise ValueError("one N per seed")

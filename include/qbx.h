@@ -164,6 +164,10 @@ int qbx_batch_plan_destroy(qbx_batch_plan_t *plan);
 typedef struct {
     uint64_t state_hi, state_lo, inc_hi, inc_lo;
 } qbx_pcg64_t;
+/* The generator states numpy's Generator(PCG64(seed)) starts from, for integer seeds
+ * 0 <= seed < 2^64 (SeedSequence hash, then PCG64 seeding; random_qubo.py:21 and tsp.py:55
+ * seed their generators so).  Host only: needs no context and no GPU. */
+int qbx_pcg64_from_seeds(int count, const uint64_t *seeds, qbx_pcg64_t *out);
 int qbx_batch_random_qubo(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *ns,
                           int64_t low, int64_t high, int dtype, const qbx_opts_t *opts, qbx_result_t *out);
 int qbx_random_qubo_read(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *ns,
@@ -173,8 +177,12 @@ int qbx_random_qubo_read(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, cons
  * are built on the device, bit for bit as the reference builds them (tsp.py:50-75,
  * 272-321: uniform cities, distances in the cities' type, tour cost plus A times the
  * constraints, folded to the upper triangle, cast to dtype); solved on the FP64 path.
- * The reference builds them with Python loops on the host.  qbx_random_tsp_read returns
- * the matrices ((n_cities^2)^2 doubles each, packed) instead of solving them. */
+ * The cities and their distances are computed in the calling process, with the C
+ * library's pow and sqrt (the functions numpy's scalar `**` and np.sqrt call: pow(x, 2)
+ * is not always the correctly rounded x*x), and staged as 21 doubles per problem; the
+ * (n_cities^2)^2 QUBO is built on the device.  The reference builds it with Python loops
+ * on the host.  qbx_random_tsp_read returns the matrices ((n_cities^2)^2 doubles each,
+ * packed) instead of solving them. */
 int qbx_batch_random_tsp(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *n_cities,
                          double low, double high, int dtype, const qbx_opts_t *opts, qbx_result_t *out);
 int qbx_random_tsp_read(qbx_ctx_t *ctx, int count, const qbx_pcg64_t *rng, const int32_t *n_cities,

@@ -66,6 +66,47 @@ def test_host_build_of_tsp_generator_matches_numpy(emu, seed, nc, low, high, dty
     assert np.array_equal(out, ref.astype(np.float64))
 
 
+def _tsp_seeds(count, dtype):
+    rng = np.random.default_rng(1 if dtype is np.float32 else 2)
+    return [int(s) for s in rng.integers(0, 2 ** 31, count)], [int(c) for c in rng.integers(4, 6, count)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_host_build_of_tsp_generator_many_seeds(emu, dtype):
+    """4096 instances of 4-5 cities: numpy's scalar `x ** 2` (tsp.py:279) is the C
+    library's pow, which is not always the correctly rounded x * x; about one square in
+    1500 differs in the last bit, so a generator that squares by multiplying fails here
+    (10 of these 4096 float32 instances)."""
+    emu.qbx_emu_random_tsp.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    bad = []
+    for seed, nc in zip(*_tsp_seeds(4096, dtype)):
+        ref = _tsp_ref(seed, nc, 0, 100, dtype)
+        out = np.empty((nc * nc, nc * nc), dtype=np.float64)
+        assert emu.qbx_emu_random_tsp(_state(seed), nc, 0.0, 100.0, CODES[dtype], out.ctypes.data) == 0
+        if not np.array_equal(out, ref.astype(np.float64)):
+            bad.append(seed)
+    assert not bad, bad[:10]
+
+
+def test_squares_by_multiplication_would_differ():
+    """The reason the generator takes squares with pow: on these coordinates numpy's
+    float32 `**` 2 and the correctly rounded product differ (if this ever stops holding, the
+    C library's powf became correctly rounded and the test above loses its teeth)."""
+    rng = np.random.default_rng(0)
+    xs = rng.uniform(-100, 100, 20000).astype(np.float32)
+    assert sum(x ** 2 != x * x for x in xs) > 0
+
+
+@pytest.mark.gpu
+def test_device_tsp_many_seeds_match_reference_build(gpu_ctx):
+    for dtype in (np.float32, np.float64):
+        seeds, ncs = _tsp_seeds(4096, dtype)
+        mats = gpu_ctx.random_tsp_read(seeds, ncs, 0, 100, dtype)
+        bad = [s for s, c, m in zip(seeds, ncs, mats) if not np.array_equal(m, _tsp_ref(s, c, 0, 100, dtype).astype(np.float64))]
+        assert not bad, (dtype, bad[:10])
+
+
 @pytest.mark.gpu
 def test_device_generator_matches_numpy(gpu_ctx):
     for dtype in (np.int16, np.int32, np.float32, np.float64):
@@ -139,3 +180,22 @@ def test_solve_batch_builds_tsp_on_device(gpu_ctx):
     for a, b in zip(dev, host):
         assert a.best_E == b.best_E and list(a.best_X) == list(b.best_X)
         assert a.ising_offset == b.ising_offset
+
+
+def test_pcg64_seeding_matches_numpy():
+    """qbx_pcg64_from_seeds (host only, no GPU) gives numpy's PCG64(seed) state for seeds
+    of one and two 32-bit words, including the edges 0, 2^32 - 1, 2^32 and 2^64 - 1."""
+    from qubo_lib_amd._native import pcg64_states
+    rng = np.random.default_rng(3)
+    seeds = list(range(300)) + [2 ** 32 - 1, 2 ** 32, 2 ** 64 - 1, 2 ** 63] \
+        + [int(s) for s in rng.integers(0, 2 ** 63, 300, dtype=np.int64)] \
+        + [int(s) for s in rng.integers(0, 2 ** 32, 300, dtype=np.int64)]
+    got = pcg64_states(seeds)
+    m = (1 << 64) - 1
+    for seed, st in zip(seeds, got):
+        s = PCG64(seed).state["state"]
+        assert (st.state_hi, st.state_lo, st.inc_hi, st.inc_lo) == \
+            (s["state"] >> 64, s["state"] & m, s["inc"] >> 64, s["inc"] & m), seed
+    big = pcg64_states([2 ** 70 + 5])[0]          # three words: numpy's SeedSequence
+    s = PCG64(2 ** 70 + 5).state["state"]
+    assert (big.state_hi << 64 | big.state_lo) == s["state"]
