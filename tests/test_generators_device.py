@@ -89,6 +89,30 @@ def test_host_build_of_tsp_generator_many_seeds(emu, dtype):
     assert not bad, bad[:10]
 
 
+TSP_MIX = [(0, 100), (-50, 50), (0, 1), (0, 1000), (-3, 7)]
+
+
+def _tsp_mix(count, seed):
+    rng = np.random.default_rng(seed)
+    return [int(s) for s in rng.integers(0, 2 ** 63, count, dtype=np.int64)], [int(c) for c in rng.integers(1, 8, count)]
+
+
+@pytest.mark.parametrize("low,high", TSP_MIX)
+def test_host_build_of_tsp_generator_all_city_counts(emu, low, high):
+    """1..7 cities, float32 and float64, several coordinate ranges, 64-bit seeds."""
+    emu.qbx_emu_random_tsp.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    for dtype in (np.float32, np.float64):
+        bad = []
+        for seed, nc in zip(*_tsp_mix(300, 1000 + 7 * low + high)):
+            ref = _tsp_ref(seed, nc, low, high, dtype)
+            out = np.empty((nc * nc, nc * nc), dtype=np.float64)
+            assert emu.qbx_emu_random_tsp(_state(seed), nc, float(low), float(high), CODES[dtype], out.ctypes.data) == 0
+            if not np.array_equal(out, ref.astype(np.float64)):
+                bad.append((seed, nc))
+        assert not bad, (dtype, bad[:5])
+
+
 def test_squares_by_multiplication_would_differ():
     """The reason the generator takes squares with pow: on these coordinates numpy's
     float32 `**` 2 and the correctly rounded product differ (if this ever stops holding, the
@@ -105,6 +129,17 @@ def test_device_tsp_many_seeds_match_reference_build(gpu_ctx):
         mats = gpu_ctx.random_tsp_read(seeds, ncs, 0, 100, dtype)
         bad = [s for s, c, m in zip(seeds, ncs, mats) if not np.array_equal(m, _tsp_ref(s, c, 0, 100, dtype).astype(np.float64))]
         assert not bad, (dtype, bad[:10])
+
+
+@pytest.mark.gpu
+def test_device_tsp_all_city_counts_match_reference_build(gpu_ctx):
+    for low, high in TSP_MIX:
+        seeds, ncs = _tsp_mix(300, 1000 + 7 * low + high)
+        for dtype in (np.float32, np.float64):
+            mats = gpu_ctx.random_tsp_read(seeds, ncs, low, high, dtype)
+            bad = [(s, c) for s, c, m in zip(seeds, ncs, mats)
+                   if not np.array_equal(m, _tsp_ref(s, c, low, high, dtype).astype(np.float64))]
+            assert not bad, (low, high, dtype, bad[:5])
 
 
 @pytest.mark.gpu
