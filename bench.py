@@ -126,7 +126,7 @@ def exhaustive(args, ctx, devinfo):
     if args.workload == "float":
         n = 34
         Q = symmetric_normal_qubo(n, args.seed)
-        workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: quantized int32 filter walk, FP64 exact re-walk"
+        workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: filter walk over quantized tables, FP64 exact re-walk"
         dtype = "f32"
     elif args.workload == "maxcut":
         n = 40
@@ -163,7 +163,7 @@ def exhaustive(args, ctx, devinfo):
     peak = devinfo["valu_peak_ops"]
     # the packed walks do two 16-bit operations per lane and instruction: their ceiling is
     # the packed 16-bit VALU rate, twice the int32 one
-    packed = r.info["walk"] in ("pair16", "pair16_rebased")
+    packed = r.info["walk"] in ("pair16", "pair16_rebased", "float_pair16")
     if packed:
         peak *= 2
     achieved = states * OPS_PER_STATE / (kernel_ms * 1e-3)

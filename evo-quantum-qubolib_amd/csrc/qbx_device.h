@@ -41,11 +41,11 @@ struct InstDev {
                              // them, or one part's (the last commit finalizes)
     const void* Q;           // source Q: int32 or double, row-major, variable order
     void* W;                 // n x LDW, bit order: int32 W followed by packed W 65537 |
-                             // double followed by int32 Wq
+                             // double followed by int32 Wq and packed coarse Wq
     void* d;                 // 64 entries: int32 | double followed by int32 dq
     void* delta;             // 2^M entries: int32 trie constants followed by the same times
                              // 65537 | FP64 Gray step constants followed by the int32 trie
-                             // constants of the quantized tables
+                             // constants of the quantized tables and their packed coarse copies
     RedDev* red;
     uint32_t* cand;          // chunks entries
     SlotDev* slots;          // chunks entries
@@ -54,17 +54,20 @@ struct InstDev {
     double qmargin;          // float kind: filter margin in quantized units
     int64_t bound;           // sum |d| + sum_{a<b} |W|  (int kind; quantized for float)
     double fbound;           // same in double (float kind)
-    int32_t path;            // int kind, set by qbx_prep: QBX_PATH_INT32 | _PAIR16 | _PAIR16_REBASED
-    int32_t slack;           // rebased pair walk: PAIR_BOUND - D, the guard's room
+    int32_t path;            // set by qbx_prep: QBX_PATH_INT32 | _PAIR16 | _PAIR16_REBASED (int kind),
+                             // QBX_PATH_FLOAT | _FLOAT_PAIR16 (float kind)
+    int32_t slack;           // rebased pair walks: PAIR_BOUND - D, the guard's room
+    int32_t qshift;          // packed float filter: coarse scale 2^-qshift of the quantized one
+    int32_t pad;
 };
 
 // Table regions of one problem (bytes), shared by the batch and serving layouts.
 __host__ __device__ inline size_t table_W_bytes(int kind, int n) {
-    return (size_t)n * LDW * (kind == KIND_I32 ? 8 : 12);
+    return (size_t)n * LDW * (kind == KIND_I32 ? 8 : 16);
 }
 __host__ __device__ inline size_t table_d_bytes(int kind) { return 64 * (kind == KIND_I32 ? 4 : 12); }
 __host__ __device__ inline size_t table_delta_bytes(int kind, int M) {
-    return (size_t)(1u << M) * (kind == KIND_I32 ? 8 : 12);
+    return (size_t)(1u << M) * (kind == KIND_I32 ? 8 : 16);
 }
 
 struct TileRange {
@@ -204,6 +207,21 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             I.bound = red_i[0];
             // finite energies whose quantized bound fits int32 (q_scale guarantees it)
             I.status = (ok && red_i[0] <= 0x7ffffffeLL) ? 0 : QBX_ERANGE;
+            // the packed filter wherever a coarse scale leaves its guard room
+            int64_t D = 0;
+            const int sh = (I.status == 0) ? qrp_shift(Wq, dq, n, M, I.L, &D) : -1;
+            I.path = sh >= 0 ? QBX_PATH_FLOAT_PAIR16 : QBX_PATH_FLOAT;
+            I.qshift = sh >= 0 ? sh : 0;
+            I.slack = sh >= 0 ? (int32_t)(PAIR_BOUND - D) : 0;
+        }
+        __syncthreads();
+        if (I.path == QBX_PATH_FLOAT_PAIR16) {
+            // coarse packed copies: rnd(v / 2^sh) in both halves (|.| < 2^15 by the guard)
+            const int sh = I.qshift;
+            uint32_t* Wpc = (uint32_t*)(Wq + (size_t)n * LDW);
+            uint32_t* dlpc = (uint32_t*)(dlq + (1 << M));
+            for (int idx = tid; idx < n * LDW; idx += blockDim.x) Wpc[idx] = pack_both((uint32_t)rnd_shr(Wq[idx], sh));
+            for (uint32_t t = tid; t < (1u << M); t += blockDim.x) dlpc[t] = pack_both((uint32_t)rnd_shr(dlq[t], sh));
         }
     }
     if (tid == 0) {
@@ -293,7 +311,7 @@ static __device__ void finalize_problem(const InstDev& I, int lane) {
         I.result->rank = brank;
         I.result->state = rank_to_state(brank, I.B);
         I.result->status = I.status;
-        I.result->path = I.kind == KIND_F64 ? QBX_PATH_FLOAT : I.path;
+        I.result->path = I.path;
     }
 }
 
@@ -448,8 +466,9 @@ struct WaveBound {
     }
 };
 
+// Two waves per SIMD (at most 256 VGPRs): the filters' latency hides behind the other wave.
 template <int M, int OMAX>
-static __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restrict__ insts,
+static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qbx_walk_f64(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
                                                    unsigned long long total,
                                                    unsigned long long* ticket, int zero) {
@@ -467,9 +486,6 @@ static __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restr
             const int32_t* Wq = (const int32_t*)(W + (size_t)n * LDW);
             const int32_t* dq = (const int32_t*)(d + 64);
             const int32_t* dsrc = (const int32_t*)(dld + (1 << M));
-            int32_t dlq[1 << M];
-#pragma unroll
-            for (int k = 0; k < (1 << M); ++k) dlq[k] = dsrc[k + (lane & zero)];
             const uint32_t t0 = chunk * I.tpc;
             const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
             // branch and bound across waves through the problem's running minimum key
@@ -477,15 +493,36 @@ static __global__ __launch_bounds__(64) void qbx_walk_f64(const InstDev* __restr
             // per-lane candidate records, lane-interleaved (entry k of lane l at k * 64 + l)
             __shared__ uint32_t qc[QCAP * 64];
             __shared__ int32_t qmn[QCAP * 64];
-            for (uint32_t tile = t0; tile < t1; ++tile) {
-                // every lane walks (the walk shuffles across the wave); a lane past the end
-                // of a small problem repeats lane 0's work and its result is dropped
-                const uint64_t lam0 = (uint64_t)tile * 64u;
-                const uint64_t lam = lam0 + (uint64_t)lane;
-                const bool active = H >= 6 || lam < (1ull << H);
-                const LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq, dq, W, d, n, L, I.B, active ? lam : lam0, dlq, dld,
-                                                            I.qscale, I.qmargin, wb, qc + lane, qmn + lane, 64);
-                if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
+            // every lane walks (the walks shuffle across the wave); a lane past the end of a
+            // small problem repeats lane 0's work and its result is dropped
+            if (I.path == QBX_PATH_FLOAT_PAIR16) {
+                const uint32_t* Wpc = (const uint32_t*)(Wq + (size_t)n * LDW);
+                const uint32_t* psrc = (const uint32_t*)(dsrc + (1 << M));
+                uint32_t dlp[1 << M];
+#pragma unroll
+                for (int k = 0; k < (1 << M); ++k) dlp[k] = psrc[k + (lane & zero)];
+                for (uint32_t tile = t0; tile < t1; ++tile) {
+                    const uint64_t lam0 = (uint64_t)tile * 64u;
+                    const uint64_t lam = lam0 + (uint64_t)lane;
+                    const bool active = H >= 6 || lam < (1ull << H);
+                    const LaneOutF64 o = walk_lane_qrp<M, OMAX>(Wq, dq, Wpc, W, d, n, L, I.B, active ? lam : lam0,
+                                                                dlp, dld, I.qscale, I.qmargin, I.qshift, I.slack,
+                                                                wb, qc + lane, qmn + lane, 64, nullptr);
+                    if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
+                }
+            } else {
+                int32_t dlq[1 << M];
+#pragma unroll
+                for (int k = 0; k < (1 << M); ++k) dlq[k] = dsrc[k + (lane & zero)];
+                for (uint32_t tile = t0; tile < t1; ++tile) {
+                    const uint64_t lam0 = (uint64_t)tile * 64u;
+                    const uint64_t lam = lam0 + (uint64_t)lane;
+                    const bool active = H >= 6 || lam < (1ull << H);
+                    const LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq, dq, W, d, n, L, I.B, active ? lam : lam0, dlq,
+                                                                dld, I.qscale, I.qmargin, wb, qc + lane, qmn + lane,
+                                                                64);
+                    if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
+                }
             }
         }
         for (int off = 32; off > 0; off >>= 1) {

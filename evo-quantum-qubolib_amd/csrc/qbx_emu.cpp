@@ -21,6 +21,8 @@ unsigned long long emu_stat_flushes = 0, emu_stat_rewalks = 0;
 static unsigned long long emu_stat_pair_lanes = 0;
 static unsigned long long emu_stat_rp_lanes = 0, emu_stat_rp_int32_groups = 0;
 static int emu_rp_slack_override = -1;
+static unsigned long long emu_stat_qrp_lanes = 0, emu_stat_qrp_int32_groups = 0;
+static int emu_qrp_slack_override = -1, emu_qrp_shift_delta = 0;
 
 namespace {
 
@@ -135,6 +137,35 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
     }
     *E_out = bE;
     *rank_out = bR;
+    // the packed filter, where qbx_prep would select it (shift and slack overridable so
+    // that tests can force fine scales and int32 groups), must give the same answer
+    int64_t D = 0;
+    int sh = qrp_shift(Wq.data(), dq.data(), n, M, L, &D);
+    if (sh >= 0 && emu_qrp_shift_delta != 0) {
+        sh = sh + emu_qrp_shift_delta < 0 ? 0 : (sh + emu_qrp_shift_delta > 30 ? 30 : sh + emu_qrp_shift_delta);
+        D = qrp_guard_D(Wq.data(), M, L, sh);
+        if (D < 0) sh = -1;
+    }
+    if (sh >= 0) {
+        int32_t slack = (int32_t)(PAIR_BOUND - D);
+        if (emu_qrp_slack_override >= 0 && emu_qrp_slack_override < slack) slack = emu_qrp_slack_override;
+        std::vector<uint32_t> Wpc(Wq.size()), dlpc(dlq.size());
+        for (size_t i = 0; i < Wq.size(); ++i) Wpc[i] = pack_both((uint32_t)rnd_shr(Wq[i], sh));
+        for (size_t i = 0; i < dlq.size(); ++i) dlpc[i] = pack_both((uint32_t)rnd_shr(dlq[i], sh));
+        double pE = __builtin_inf();
+        uint64_t pR = ~0ull;
+        HostBound hp{__builtin_inf()};
+        for (uint64_t lam = 0; lam < lanes; ++lam) {
+            uint32_t fallbacks = 0;
+            LaneOutF64 o = walk_lane_qrp<M, OMAX>(Wq.data(), dq.data(), Wpc.data(), W.data(), d.data(), n, L, B,
+                                                  lam, dlpc.data(), dld.data(), scale, qm, sh, slack, hp, buf_c,
+                                                  buf_m, 1, &fallbacks);
+            if (o.E < pE || (o.E == pE && o.rank < pR)) { pE = o.E; pR = o.rank; }
+            ++emu_stat_qrp_lanes;
+            emu_stat_qrp_int32_groups += fallbacks;
+        }
+        if (!(pE == bE && pR == bR)) return -7;
+    }
     return 0;
 }
 
@@ -146,7 +177,8 @@ extern "C" {
 // -4 float energies out of range; -5 the packed pair walk's lane key differs from the
 // int32 walk's (checked for every lane when the problem's bound selects the pair walk);
 // -6 the rebased pair walk's lane key differs (checked for every lane when its guard
-// constant D < 15359).
+// constant D < 15359); -7 the packed float filter's answer differs from the int32
+// filter's (checked whenever qbx_prep would select it).
 int qbx_emu_solve_i32(const int32_t* Q, int n, int B, int L, int M, int64_t* E, uint64_t* rank,
                       uint64_t* state) {
     int rc;
@@ -210,6 +242,21 @@ void qbx_emu_take_stats(uint64_t* flushes, uint64_t* rewalks) {
     *flushes = emu_stat_flushes;
     *rewalks = emu_stat_rewalks;
     emu_stat_flushes = emu_stat_rewalks = 0;
+}
+
+// packed float filter: lanes walked and groups walked in int32 since the last call
+void qbx_emu_take_qrp_stats(uint64_t* lanes, uint64_t* int32_groups) {
+    *lanes = emu_stat_qrp_lanes;
+    *int32_groups = emu_stat_qrp_int32_groups;
+    emu_stat_qrp_lanes = emu_stat_qrp_int32_groups = 0;
+}
+
+// packed float filter overrides: a smaller guard slack (-1: PAIR_BOUND - D_c, as the
+// device uses) and a change of the chosen coarse shift (0: as the device chooses; a finer
+// scale fails the guard more often, a coarser one passes more candidates)
+void qbx_emu_set_qrp(int32_t slack, int32_t shift_delta) {
+    emu_qrp_slack_override = slack;
+    emu_qrp_shift_delta = shift_delta;
 }
 
 // lanes whose pair-walk key was checked since the last call

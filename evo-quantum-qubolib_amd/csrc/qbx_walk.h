@@ -1074,11 +1074,12 @@ struct LaneOutF64 {
     uint64_t rank;
 };
 
-// 2^s: the largest power of two that keeps the quantized energy bound within int32
-// (sum |rint(v 2^s)| <= bound 2^s + (number of terms) / 2 <= 2^31 - 2).
+// 2^s: the largest power of two that keeps the quantized energy bound within 2^29
+// (sum |rint(v 2^s)| <= bound 2^s + (number of terms) / 2 <= 2^29 - 2), so that the
+// packed filter's differences of two quantized energies fit int32 with room to spare.
 QBX_HD double q_scale(double bound, int n) {
     const double half_terms = 0.25 * (double)n * (double)(n + 1);
-    const double room = 2147483646.0 - half_terms - 1.0;
+    const double room = 536870910.0 - half_terms - 1.0;
     if (!(bound > 0.0)) return 1.0;
     const double r = room / bound;
     if (!(r < 0x1p1000)) return 0x1p1000;   // tiny entries: any large scale is exact enough
@@ -1190,24 +1191,44 @@ struct QCtx {    // what a flush needs to rebuild and walk a block exactly
     const double* W;
     const double* d;
     const double* dld;
-    int n, O, SBk, B, parp;
-    uint64_t p, laneRank;
-    uint32_t pmaskS;
+    int n, L, B;
+    uint64_t lam;    // the lane; its prefix, rank bits and block-key mask are derived per flush
+    QBX_HD_MEMBER void set(const double* W_, const double* d_, const double* dld_, int n_, int L_, int B_,
+                           uint64_t lam_) {
+        W = W_; d = d_; dld = dld_; n = n_; L = L_; B = B_; lam = lam_;
+    }
 };
 
 // Drop the records the filter no longer passes; with `walk`, walk the rest in FP64 and
-// empty the buffer.
-template <int M>
+// empty the buffer.  PACKED records (the packed float filter) name a packed block c and
+// the halves h that passed, as (c << 2) | mask; half h is block (c << 1) | (h ^ (c & 1))
+// of the int32 walk's numbering.
+template <int M, bool PACKED = false>
 QBX_HD void q_compact(QBuf& b, const QLane& q, const QCtx<M>& x, bool walk, double& bestE, uint64_t& bestR) {
+    const int O = x.L - M;
+    const uint64_t p = lane_prefix(x.lam, x.L, x.B);
+    const uint64_t laneRank = shl64(x.lam, x.L);
+    const int SBk = (x.B - M) < 0 ? 0 : ((x.B - M) > O ? O : (x.B - M));
+    const int parp = parity64(p);
+    const uint32_t pmaskS = (parp && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
     int keep = 0;
     for (int k = 0; k < b.count; ++k) {
         const int32_t m = b.m[k * b.stride];
         if (m > q.thr) continue;
         const uint32_t c = b.c[k * b.stride];
         if (walk) {
-            QBX_STAT(++emu_stat_rewalks);
-            block_exact_f64<M>(x.W, x.d, x.n, x.p, x.O, c, x.laneRank, x.SBk, x.pmaskS, x.B, x.parp, x.dld,
-                               bestE, bestR);
+            if (PACKED) {
+                const uint32_t cp = c >> 2;
+                for (uint32_t h = 0; h < 2u; ++h) {
+                    if (!((c >> h) & 1u)) continue;
+                    QBX_STAT(++emu_stat_rewalks);
+                    block_exact_f64<M>(x.W, x.d, x.n, p, O, (cp << 1) | (h ^ (cp & 1u)), laneRank, SBk, pmaskS,
+                                       x.B, parp, x.dld, bestE, bestR);
+                }
+            } else {
+                QBX_STAT(++emu_stat_rewalks);
+                block_exact_f64<M>(x.W, x.d, x.n, p, O, c, laneRank, SBk, pmaskS, x.B, parp, x.dld, bestE, bestR);
+            }
         } else {
             b.c[keep * b.stride] = c;
             b.m[keep * b.stride] = m;
@@ -1279,16 +1300,11 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
                                 uint32_t* buf_c, int32_t* buf_m, int buf_stride) {
     static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
     const int O = L - M;
-    const uint64_t p = lane_prefix(lam, L, B);
     LaneI32<M, OMAX> s;
     lane_setup_i32<M, OMAX>(s, Wq, dq, n, L, B, lam);
 
     QCtx<M> x;
-    x.W = W; x.d = d; x.dld = dld; x.n = n; x.O = O; x.B = B; x.p = p;
-    x.SBk = (B - M) < 0 ? 0 : ((B - M) > O ? O : (B - M));
-    x.parp = parity64(p);
-    x.pmaskS = (x.parp && O > x.SBk) ? (lowmask32(O - x.SBk) << x.SBk) : 0u;
-    x.laneRank = shl64(lam, L);
+    x.set(W, d, dld, n, L, B, lam);
 
     double bound = src.refresh();
     double bestE = __builtin_inf();
@@ -1332,6 +1348,290 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
         trans_dyn_i32<M, OMAX>(s, Wq + (M + i) * LDW, i, up);
     }
     q_compact<M>(b, q, x, true, bestE, bestR);
+    LaneOutF64 out;
+    out.E = bestE;
+    out.rank = bestR;
+    return out;
+}
+
+// ------------------------------------ float problems: packed 16-bit filter (rebased pairs)
+// The filter of walk_lane_q32 on the rebased pair walk.  The lane state is the exact int32
+// state of the quantized tables (bases E0 and E0 + up of the two blocks of a packed word,
+// inner fields f, upper fields u: LaneRP, advanced by the integer rebased walk's
+// transitions on Wq).  For each group of 16 packed blocks the inner fields are rounded to
+// a coarse scale, 2^-sh of the quantized one, and both blocks of each word are walked as
+// 16-bit coarse energies relative to their bases (PAIR_BIAS added), with coarse rows
+// pc = rnd(Wq / 2^sh) for the group's transitions and coarse trie constants
+// cc = rnd(c / 2^sh) (rnd: to nearest, halves up).
+//
+// Error bound.  For a state y of a block with exact quantized relative energy
+// R(y) = sum_{k in y} f_k + sum_{trie path of y} c, the coarse value R_c(y) differs from
+// R(y) / 2^sh by at most
+//   * 1/2 per field of y for its rounding at the group's start (the high block's field
+//     f_k + Wq_Mk is rounded as a whole),
+//   * 1/2 per group bit flipped since the group's start, per field (a bit flipped twice
+//     added and took away the same coarse row): at most 4 x 1/2 = 2,
+//   * 1/2 per trie constant on y's path (at most M of them),
+// so |R(y) / 2^sh - R_c(y)| <= M (1/2 + 2) + M / 2 = 3M <= QRP_ERR.  A block whose coarse
+// minimum is m (unbiased) therefore holds no state with Eq < E0 + (m - QRP_ERR) 2^sh (the
+// filter tests this bound against the threshold of walk_lane_q32) and holds a state with
+// Eq <= E0 + (m + QRP_ERR) 2^sh (what the lane minimum takes).
+//
+// Range.  Every coarse energy a packed group materialises is a state's, and
+//     |R_c(y)| <= sum_k |fc_k(group start)| + D_c,
+//     D_c = ceil(sum_{k<l<M} |Wq_kl| / 2^sh) + 4 + sum_k sum_{i<4} |pc_{M+1+i,k}|
+//           + sum_k (|rnd(Wq_Mk / 2^sh)| + 2),
+// so the guard sum_k |fc_k| <= PAIR_BOUND - D_c keeps every half in [0x0401, 0x7bff]; a
+// wave any of whose lanes fails it walks the group with coarse values in int32 (fields
+// rounded per block: error <= M / 2 + M / 2).  Float problems are quantized for
+// |Eq| < 2^29 (q_scale), and thresholds are capped at QRP_THR_MAX, so threshold - base
+// fits int32.
+constexpr int32_t QRP_ERR = 21;
+constexpr int32_t QRP_THR_MAX = 0x20100000;   // above every quantized energy (< 2^29)
+
+// v / 2^sh rounded to nearest, halves up (|v| <= 2^30, 0 <= sh <= 30)
+QBX_HD int32_t rnd_shr(int32_t v, int sh) { return sh == 0 ? v : ((v + (1 << (sh - 1))) >> sh); }
+QBX_HD int32_t clamp_i32(int64_t v) {
+    return v > 0x7fffffffLL ? 0x7fffffff : (v < -0x80000000LL ? (int32_t)0x80000000 : (int32_t)v);
+}
+
+// D_c of the guard for shift sh, or -1 when a coarse table value could leave 15 bits.
+QBX_HD int64_t qrp_guard_D(const int32_t* Wq, int M, int L, int sh) {
+    int64_t cin = 0, dg = 0, dp = 0;
+    for (int k = 0; k < M; ++k)
+        for (int l = k + 1; l < M; ++l) {
+            const int64_t w = Wq[k * LDW + l];
+            cin += w < 0 ? -w : w;
+        }
+    cin = ((cin + ((1ll << sh) - 1)) >> sh) + 4;
+    for (int k = 0; k < M; ++k) {
+        const int64_t wm = rnd_shr(Wq[M * LDW + k], sh);
+        dp += (wm < 0 ? -wm : wm) + 2;
+        for (int i = 0; i < GROUP_BITS; ++i) {
+            const int J = M + 1 + i;
+            if (J >= L) break;
+            const int64_t w = rnd_shr(Wq[J * LDW + k], sh);
+            dg += w < 0 ? -w : w;
+        }
+    }
+    const int64_t D = cin + dg + dp;
+    return D < PAIR_BOUND ? D : -1;
+}
+
+// Shift of the packed float filter: the smallest sh (finest coarse scale) for which the
+// guard constant leaves room for the expected inner fields (the estimate of rp_worth_it,
+// on the quantized tables), or -1 (then the int32 filter runs).  *D_out: D_c at sh.
+QBX_HD int qrp_shift(const int32_t* Wq, const int32_t* dq, int n, int M, int L, int64_t* D_out) {
+    if (L <= M + 1) return -1;
+    double S = 0.0, Din = 0.0;
+    for (int k = 0; k < M; ++k) {
+        double mean = dq[k], var = 0.0;
+        for (int j = M; j < n; ++j) {
+            const double w = Wq[k * LDW + j];
+            mean += 0.5 * w;
+            var += 0.25 * w * w;
+        }
+        S += sqrt(mean * mean + var);
+        for (int l = k + 1; l < M; ++l) Din += fabs((double)Wq[k * LDW + l]);
+        for (int i = 0; i <= GROUP_BITS && M + i < n; ++i) Din += fabs((double)Wq[(M + i) * LDW + k]);
+    }
+    // coarse units scale as 2^-sh: start just below the estimate and walk up
+    const double need = (Din + 1.5 * S) / (double)(PAIR_BOUND - 64 - 4 * M);
+    int sh = 0;
+    while (sh < 30 && (double)(1ll << sh) < need) ++sh;
+    for (sh = sh > 1 ? sh - 1 : 0; sh <= 30; ++sh) {
+        const int64_t D = qrp_guard_D(Wq, M, L, sh);
+        if (D < 0) continue;
+        if ((double)D + 1.5 * S / (double)(1ll << sh) <= (double)PAIR_BOUND) {
+            *D_out = D;
+            return sh;
+        }
+    }
+    return -1;
+}
+
+// Candidate record of packed block c from the minima of its halves (rare path).
+template <int M, int OMAX>
+QBX_HD void qrp_record(const LaneRP<M, OMAX>& s, uint32_t c, const int32_t* m, const int32_t* t, int32_t bias,
+                       int sh, QLane& q, QBuf& b) {
+    uint32_t mask = 0;
+    int64_t lb = 0x7fffffffffffffffLL;
+    for (int h = 0; h < 2; ++h) {
+        if (m[h] > t[h]) continue;
+        mask |= 1u << h;
+        const int64_t base = (int64_t)s.E0 + (h ? (int64_t)s.up : 0);
+        const int64_t lo = base + ((int64_t)(m[h] - bias - QRP_ERR) << sh);
+        const int32_t ub = clamp_i32(base + ((int64_t)(m[h] - bias + QRP_ERR) << sh));
+        lb = lo < lb ? lo : lb;
+        q.laneMin = ub < q.laneMin ? ub : q.laneMin;
+    }
+    q_update_thr(q);
+    q.thr = q.thr < QRP_THR_MAX ? q.thr : QRP_THR_MAX;
+    b.c[b.count * b.stride] = (c << 2) | mask;
+    b.m[b.count * b.stride] = clamp_i32(lb);
+    ++b.count;
+}
+
+template <int M, int OMAX, int J>
+QBX_HD bool group_step_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ Wq,
+                           const uint32_t* __restrict__ Wpc, const uint32_t* dl, uint32_t c0, uint32_t nblk,
+                           int sh, QLane& q, QBuf& b) {
+    const uint32_t c = c0 + (uint32_t)J;
+    if (J > 0) {
+        if (c >= nblk) return false;
+        constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
+        const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
+                                             : (((c0 >> GROUP_BITS) & 1u) == 0u);
+        // (no row prefetch: its registers would cost this kernel its second wave)
+        WRowRP<M> row;
+        load_wrow_rp<M, i>(row, Wq, Wpc);
+        trans_row_rp<M, OMAX, i>(s, F, row, up);
+    }
+    const uint32_t mp = block_min_pair<M>((uint32_t)PAIR_BIAS * 65537u, F, dl);
+    // half h passes when E0_h + (m_h - PAIR_BIAS - QRP_ERR) 2^sh <= thr, i.e. when
+    // m_h <= floor((thr - E0_h) / 2^sh) + PAIR_BIAS + QRP_ERR
+    int32_t m[2], t[2];
+    m[0] = (int32_t)(mp & 0xffffu);
+    m[1] = (int32_t)(mp >> 16);
+    t[0] = ((q.thr - s.E0) >> sh) + (PAIR_BIAS + QRP_ERR);
+    t[1] = ((q.thr - (s.E0 + s.up)) >> sh) + (PAIR_BIAS + QRP_ERR);
+    if (__builtin_expect(m[0] <= t[0] || m[1] <= t[1], 0)) qrp_record<M, OMAX>(s, c, m, t, PAIR_BIAS, sh, q, b);
+    return true;
+}
+
+template <int M, int OMAX, int... Js>
+QBX_HD void group_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ Wq,
+                      const uint32_t* __restrict__ Wpc, const uint32_t* dl, uint32_t c0, uint32_t nblk, int sh,
+                      QLane& q, QBuf& b, std::integer_sequence<int, Js...>) {
+    (void)(group_step_qrp<M, OMAX, Js>(s, F, Wq, Wpc, dl, c0, nblk, sh, q, b) && ...);
+}
+
+// The group at c0 with coarse values in int32 (a wave whose guard failed): the fine state
+// advances block by block as in the integer fallback, and each block's halves get their
+// fields rounded from the fine ones.
+template <int M, int OMAX>
+QBX_HD void group_qrp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ Wq, const uint32_t* dl,
+                            uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b) {
+    const int32_t* WM = Wq + M * LDW;
+    const uint32_t cend = (c0 + (1u << GROUP_BITS) < nblk) ? c0 + (1u << GROUP_BITS) : nblk;
+#pragma unroll 1
+    for (uint32_t c = c0; c < cend; ++c) {
+        if (c > c0) {
+            const int i = ctz32(c - c0);
+            const bool up = (((c ^ (c >> 1)) >> i) & 1u) != 0u;
+            trans_grp_rp<M, OMAX>(s, Wq + (M + 1 + i) * LDW, i, up);
+        }
+        int32_t m[2], t[2];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            int32_t fh[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k) fh[k] = rnd_shr((int32_t)((uint32_t)s.f[k] + (h ? (uint32_t)WM[k] : 0u)), sh);
+            int32_t mn = 0;
+            uint32_t z = 0u;
+            QBX_OPAQUE(z);
+            utrie_even<M, 0u>(0u, fh, dl, z, mn);
+            const int32_t base = (int32_t)((uint32_t)s.E0 + (h ? (uint32_t)s.up : 0u));
+            m[h] = mn;
+            t[h] = ((q.thr - base) >> sh) + QRP_ERR;
+        }
+        if (m[0] <= t[0] || m[1] <= t[1]) qrp_record<M, OMAX>(s, c, m, t, 0, sh, q, b);
+    }
+}
+
+// Walk one lane of a float problem with the packed filter (tables: Wq, dq and the coarse
+// packed Wpc, dlpc of shift sh; slack = PAIR_BOUND - D_c); same contract and FP64 re-walks
+// as walk_lane_q32.  groups_int32 (nullable) counts the groups walked in int32.
+template <int M, int OMAX, class Bound>
+QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
+                                const uint32_t* __restrict__ Wpc, const double* __restrict__ W,
+                                const double* __restrict__ d, int n, int L, int B, uint64_t lam,
+                                const uint32_t* dlpc, const double* __restrict__ dld, double scale, double qm,
+                                int sh, int32_t slack, Bound& src, uint32_t* buf_c, int32_t* buf_m,
+                                int buf_stride, uint32_t* groups_int32) {
+    static_assert(OMAX > GROUP_BITS, "OMAX must cover the unrolled transition bits");
+    const int O = L - M;
+    LaneRP<M, OMAX> s;
+    lane_setup_rp<M, OMAX>(s, Wq, dq, n, L, B, lam);
+
+    QCtx<M> x;
+    x.set(W, d, dld, n, L, B, lam);
+
+    double bound = src.refresh();
+    double bestE = __builtin_inf();
+    uint64_t bestR = ~0ull;
+    QLane q;
+    q.laneMin = 0x7fffffff;
+    q.two_qm = (int32_t)(2.0 * qm);
+    q.thrB = q_threshold(bound, scale, qm);
+    q_update_thr(q);
+    q.thr = q.thr < QRP_THR_MAX ? q.thr : QRP_THR_MAX;
+    QBuf b{buf_c, buf_m, buf_stride, 0};
+
+    const int32_t* WM = Wq + M * LDW;
+    const int32_t* W3 = Wq + (M + GROUP_BITS) * LDW;    // row of group bit GROUP_BITS - 1
+    const uint32_t nblk = 1u << (O - 1);
+    for (uint32_t c0 = 0;;) {
+        int32_t fc[M];
+        uint32_t g = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            fc[k] = rnd_shr(s.f[k], sh);
+            const uint32_t a = (uint32_t)(fc[k] < 0 ? -fc[k] : fc[k]);
+            g += a < 0x10000u ? a : 0x10000u;
+        }
+        bool ok = g <= (uint32_t)slack;
+#if defined(__HIP_DEVICE_COMPILE__)
+        ok = __all(ok);
+#endif
+        if (__builtin_expect(ok, 1)) {
+            uint32_t F[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+                F[k] = (uint32_t)fc[k] + ((uint32_t)rnd_shr((int32_t)((uint32_t)s.f[k] + (uint32_t)WM[k]), sh) << 16);
+            group_qrp<M, OMAX>(s, F, Wq, Wpc, dlpc, c0, nblk, sh, q, b,
+                               std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
+            // the fine fields follow the group's net change: a full group flips group bit
+            // GROUP_BITS - 1 once (every lower bit an even number of times); after a partial
+            // group (the lane's last) they are not used again
+            if (c0 + (1u << GROUP_BITS) <= nblk) {
+                const bool up = ((c0 >> GROUP_BITS) & 1u) == 0u;
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    s.f[k] = (int32_t)(up ? (uint32_t)s.f[k] + (uint32_t)W3[k] : (uint32_t)s.f[k] - (uint32_t)W3[k]);
+            }
+        } else {
+            group_qrp_int32<M, OMAX>(s, Wq, dlpc, c0, nblk, sh, q, b);
+            if (groups_int32) ++*groups_int32;
+        }
+        if (b.count > QCAP - (1 << GROUP_BITS)) {   // room for one more group of records
+            q_compact<M, true>(b, q, x, false, bestE, bestR);
+            if (b.count > QCAP - (1 << GROUP_BITS)) {
+                QBX_STAT(++emu_stat_flushes);
+                q_compact<M, true>(b, q, x, true, bestE, bestR);
+            }
+        }
+        if ((c0 & ((QSYNC << GROUP_BITS) - 1u)) == 0u) {
+            double mine = bestE;
+            if (q.laneMin != 0x7fffffff) {
+                const double ub = ((double)q.laneMin + qm) / scale;
+                mine = ub < mine ? ub : mine;
+            }
+            src.publish(mine);
+            const double b2 = src.refresh();
+            bound = b2 < bound ? b2 : bound;
+            const double bb = bestE < bound ? bestE : bound;
+            q.thrB = q_threshold(bb, scale, qm);
+            q_update_thr(q);
+            q.thr = q.thr < QRP_THR_MAX ? q.thr : QRP_THR_MAX;
+        }
+        c0 += 1u << GROUP_BITS;
+        if (c0 >= nblk) break;
+        const int i = ctz32(c0);
+        const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
+        trans_dyn_rp<M, OMAX>(s, Wq + (M + 1 + i) * LDW, i, up);
+    }
+    q_compact<M, true>(b, q, x, true, bestE, bestR);
     LaneOutF64 out;
     out.E = bestE;
     out.rank = bestR;

@@ -50,7 +50,7 @@ class Opts(ctypes.Structure):
 
 
 # qbx_result_t.path: the walk that ran (include/qbx.h QBX_PATH_*)
-PATHS = {0: "int32", 1: "pair16", 2: "float", 3: "pair16_rebased"}
+PATHS = {0: "int32", 1: "pair16", 2: "float", 3: "pair16_rebased", 4: "float_pair16"}
 PAIR_BOUND = 15359       # QBX_PAIR_BOUND: integer bound up to which the packed pair walk runs
 
 

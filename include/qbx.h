@@ -69,11 +69,15 @@ typedef struct {
  * QBX_PATH_PAIR16); other integer problems whose energy differences inside a walk block
  * are expected to fit 16 bits do the same relative to each block's base, with a per-group
  * guard and int32 groups where it fails (QBX_PATH_PAIR16_REBASED); the rest walk in int32.
- * All give the same answers.  Float problems take the quantized filter with FP64 re-walks. */
+ * All give the same answers.  Float problems take a filter over quantized tables with FP64
+ * re-walks of the blocks that pass it: in int32 (QBX_PATH_FLOAT), or, where a coarse
+ * scale of the quantized block-relative energies fits 16 bits, on the packed block-relative
+ * walk (QBX_PATH_FLOAT_PAIR16).  Both give the same answers. */
 #define QBX_PATH_INT32 0
 #define QBX_PATH_PAIR16 1
 #define QBX_PATH_FLOAT 2
 #define QBX_PATH_PAIR16_REBASED 3
+#define QBX_PATH_FLOAT_PAIR16 4
 #define QBX_PAIR_BOUND 15359
 
 typedef struct {

@@ -141,6 +141,44 @@ def test_float_filter_walk(emu, M):
 
 
 
+def take_qrp(emu):
+    lanes, groups = ctypes.c_uint64(), ctypes.c_uint64()
+    emu.qbx_emu_take_qrp_stats(ctypes.byref(lanes), ctypes.byref(groups))
+    return lanes.value, groups.value
+
+
+@pytest.mark.parametrize("M", [1, 4, 6, 7])
+@pytest.mark.parametrize("slack,shift_delta", [(-1, 0), (0, 0), (-1, -3), (-1, 3), (-1, 6), (40, 0)])
+def test_packed_float_filter(emu, M, slack, shift_delta):
+    """The packed float filter (coarse 16-bit block-relative energies over the quantized
+    tables) gives the int32 filter's answer exactly (the emulation returns -7 otherwise)
+    and the FP64 brute force's; with the guard's slack cut (0: every group in int32; 40:
+    some) and at finer (-3: more int32 groups) and coarser (+3: more candidates) scales."""
+    emu.qbx_emu_set_qrp.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    emu.qbx_emu_take_qrp_stats.argtypes = [P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    emu.qbx_emu_set_qrp(slack, shift_delta)
+    try:
+        rng = np.random.default_rng(61 + M)
+        take_qrp(emu)
+        lanes = groups = 0
+        for trial in range(10):
+            n = int(rng.integers(M + 6, 17)) if M + 6 < 17 else 16
+            Q = _float_case(rng, n, trial) if trial % 2 else rng.standard_normal((n, n))
+            B = [min(n, 20), 0, n // 2][trial % 3]
+            E, s = oracle.brute_f64(Q, B)
+            for L in sorted({M + 2, min(n, M + 5), min(n, M + 6), min(n, M + 10)}):
+                E2, s2 = run_f64(emu, Q, B, L, M)
+                assert s2 == s and E2 == pytest.approx(E, rel=0, abs=1e-13 * np.abs(Q).sum() + 1e-300), \
+                    (n, B, L, M, trial)
+                got = take_qrp(emu)
+                lanes, groups = lanes + got[0], groups + got[1]
+        assert lanes > 0
+        if slack == 0:
+            assert groups > 0
+    finally:
+        emu.qbx_emu_set_qrp(-1, 0)
+
+
 @pytest.mark.parametrize("M", [1, 4, 6, 7])
 def test_pair_walk_matches_int32_walk(emu, M):
     """Problems with bound <= 15359 take the packed pair walk (two states per 32-bit word).
