@@ -1,0 +1,75 @@
+"""The packed float filter on the GPU: float problems walk coarse 16-bit block-relative
+energies over their quantized tables (result path "float_pair16") and re-walk the blocks
+that pass in FP64.  Answers must be the FP64 brute force's (state exactly, energy to FP64
+summation order), for every inner width and walk width; at sizes beyond the brute force,
+quarter-integer problems must give the integer walk's answer on four times the matrix
+(energies are exact in FP64 there, so E and the tie-ruled state must agree exactly).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import random_instance
+from qubo_lib_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def flush(*a):
+    print(*a, flush=True)
+
+
+def close(E, ref, Q):
+    return abs(E - ref) <= 1e-13 * np.abs(Q).sum() + 1e-300
+
+
+@pytest.mark.parametrize("inner", [7, 6, 4, 1])
+def test_packed_float_filter_vs_oracle(gpu_ctx, inner):
+    rng = np.random.default_rng(970 + inner)
+    omax = {7: 13, 6: 14, 4: 16, 1: 19}[inner]
+    for trial in range(8):
+        n = int(rng.integers(max(inner + 3, 14), 23))
+        Q = rng.standard_normal((n, n))
+        if trial % 4 == 1:
+            Q = Q.astype(np.float32).astype(np.float64)
+        if trial % 4 == 2:      # near-ties: quarter units plus a 1e-9 perturbation
+            Q = np.round(Q * 4) / 4 + rng.standard_normal((n, n)) * 1e-9
+        B = [None, 0, int(rng.integers(0, n + 1))][trial % 3]
+        E_ref, s_ref = oracle.brute_f64(Q, B)
+        for L in sorted({inner + 2, min(n, inner + 5), min(n, inner + omax)}):
+            flush("float pair", inner, trial, n, B, L)
+            got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=inner)
+            assert got.info["walk"] == "float_pair16", (inner, trial, n, L, got.info["walk"])
+            assert got.state == s_ref and close(got.E, E_ref, Q), (inner, trial, n, B, L)
+
+
+def test_packed_float_filter_matches_int_walk_large(gpu_ctx):
+    """n = 30 .. 34: Q/4 of int16-range, ternary and MaxCut QUBOs on the float path against
+    Q on the integer walk: same state, E exactly a quarter."""
+    rng = np.random.default_rng(971)
+    for n, kind in ((30, "ternary"), (32, "maxcut"), (34, "int16")):
+        if kind == "int16":
+            U = rng.integers(-127, 128, (n, n))
+            Q = np.triu(U) + np.triu(U, 1).T
+        else:
+            Q = random_instance(rng, n, kind)
+        for B in (None, 0):
+            flush("float pair large", n, kind, B)
+            a = engine.solve_matrix(Q.astype(np.float64) / 4, tie_B=B)
+            b = engine.solve_matrix(Q, tie_B=B)
+            assert a.info["walk"] == "float_pair16", a.info["walk"]
+            assert (a.E * 4, a.state) == (b.E, b.state), (n, kind, B)
+
+
+def test_packed_float_filter_in_batches_and_shares(gpu_ctx):
+    rng = np.random.default_rng(972)
+    Qs = [rng.standard_normal((n, n)) for n in rng.integers(10, 21, 30)]
+    got = engine.solve_matrices(Qs)
+    for i, (Q, r) in enumerate(zip(Qs, got)):
+        E_ref, s_ref = oracle.brute_f64(Q)
+        assert r.state == s_ref and close(r.E, E_ref, Q), i
+    Q = rng.standard_normal((26, 26))
+    full = engine.solve_matrix(Q, walk_bits=14)
+    assert full.info["walk"] == "float_pair16"
+    merged = engine.merge_parts([engine.solve_part(Q, k, 8, walk_bits=14) for k in range(8)])
+    assert (merged.E, merged.state) == (full.E, full.state)
