@@ -87,6 +87,20 @@ def test_methods_agree_medium():
         assert oracle.energy(Q, a[1]) == a[0]
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_cpu_qbf_config_n24(seed):
+    """The reference's CPU-runnable configuration: one symmetric int32 RandomQUBO, n = 24,
+    entries uniform in [-100, 100], 2^24 states.  The numpy QBF restatement (B = 20, so
+    A = 4 prefix steps of 2^20 states: the timed CPU baseline's algorithm) and the C QBF
+    restatement reproduce the independent brute force's golden (E, argmin)."""
+    from qubo_lib_amd.problems.generators import symmetric_int_qubo
+    g = json.loads((GOLDEN / "oracle_goldens.json").read_text())[f"sym_int32_n24_s{seed}"]
+    Q = symmetric_int_qubo(24, seed, -100, 100, np.int32)
+    assert qbf_numpy.solve(Q, 20) == (g["E"], g["state"])
+    assert oracle.qbf_i64(Q, 20) == (g["E"], g["state"])
+    assert oracle.energy(Q, g["state"]) == g["E"]
+
+
 def test_maxcut_test_known_answer():
     from qubo_lib_amd.problems import MaxCut
     m = MaxCut("test")
