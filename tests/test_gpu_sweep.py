@@ -67,3 +67,35 @@ def test_packed_walks_sweep(gpu_ctx):
         assert (got.E, got.state) == oracle.brute_i64(Q, B), ("packed sweep", trial, n, B, L, M, got.info["walk"])
         seen[got.info["walk"]] = seen.get(got.info["walk"], 0) + 1
     assert all(seen.get(w, 0) >= 20 for w in ("int32", "pair16", "pair16_rebased")), seen
+
+
+def test_packed_float_filter_sweep(gpu_ctx):
+    """The float filters side by side: N(0,1), float32-cast, near-tie (quarter units plus
+    1e-9 noise), tiny (1e-150) and one-huge-entry problems, random tie widths, variants and
+    walk widths (the packed filter from L = M + 2, the int32 filter at L <= M + 1); every
+    answer against the FP64 brute force."""
+    rng = np.random.default_rng(2027)
+    seen = {}
+    for trial in range(150):
+        n = int(rng.integers(10, 23))
+        Q = rng.standard_normal((n, n))
+        kind = trial % 5
+        if kind == 1:
+            Q = Q.astype(np.float32).astype(np.float64)
+        elif kind == 2:
+            Q = np.round(Q * 4) / 4 + rng.standard_normal((n, n)) * 1e-9
+        elif kind == 3:
+            Q = Q * 1e-150
+        elif kind == 4:
+            Q = Q * 1e-3
+            Q[0, n - 1] = 1e6
+        cands = [v for v in VARIANTS if v[0] + 1 < n]
+        M, OMAX = cands[trial % len(cands)]
+        L = int(rng.integers(M, min(n, M + OMAX) + 1))
+        B = int(rng.integers(0, n + 1)) if trial % 3 else None
+        E, s = oracle.brute_f64(Q, B)
+        got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=M)
+        assert got.state == s and abs(got.E - E) <= 1e-13 * np.abs(Q).sum() + 1e-300, \
+            ("float sweep", trial, n, kind, B, L, M, got.info["walk"])
+        seen[got.info["walk"]] = seen.get(got.info["walk"], 0) + 1
+    assert seen.get("float_pair16", 0) >= 100 and seen.get("float", 0) >= 5, seen
