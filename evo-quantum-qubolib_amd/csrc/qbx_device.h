@@ -200,6 +200,12 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         // trie constants of the filter from the quantized couplings (exact int32 sums)
         for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
             dlq[t] = (int32_t)trie_from_table<int32_t, long long>(Wq, M, t);
+        // the rows the packed filter's shift choice reads (inner, pair and group bits), in
+        // LDS: thread 0 walks them serially
+        __shared__ int32_t wq_head[(7 + 1 + GROUP_BITS) * LDW];
+        const int head_rows = n < M + 1 + GROUP_BITS ? n : M + 1 + GROUP_BITS;
+        for (int idx = tid; idx < head_rows * LDW; idx += blockDim.x) wq_head[idx] = Wq[idx];
+        __syncthreads();
         if (tid == 0) {
             I.fbound = fb;
             I.qscale = scale;
@@ -209,7 +215,7 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             I.status = (ok && red_i[0] <= 0x7ffffffeLL) ? 0 : QBX_ERANGE;
             // the packed filter wherever a coarse scale leaves its guard room
             int64_t D = 0;
-            const int sh = (I.status == 0) ? qrp_shift(Wq, dq, n, M, I.L, &D) : -1;
+            const int sh = (I.status == 0) ? qrp_shift(wq_head, dq, n, M, I.L, &D) : -1;
             I.path = sh >= 0 ? QBX_PATH_FLOAT_PAIR16 : QBX_PATH_FLOAT;
             I.qshift = sh >= 0 ? sh : 0;
             I.slack = sh >= 0 ? (int32_t)(PAIR_BOUND - D) : 0;
