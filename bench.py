@@ -9,7 +9,8 @@ Q is staged once per solve (3.2 KB) and the answer (16 B) read back.
     python bench.py [--steps K] [--warmup W] [--workload exhaustive|float|batch|serve|copy]
                     [--no-cpu-baseline]
 
-Other workloads: `float` (n = 34 N(0,1) float32 QUBO, FP64-exact argmin), `batch` (4096
+Other workloads: `float` (n = 34 N(0,1) float32 QUBO, FP64-exact argmin; `--float-n` for
+another size), `batch` (4096
 problems of uneven n in one launch vs one launch per problem), `serve` (Poisson request
 stream through the continuous-batching server vs sequential plain solves), `copy` (a device
 copy of known size, to calibrate the FETCH_SIZE / WRITE_SIZE counters).
@@ -124,7 +125,7 @@ def exhaustive(args, ctx, devinfo):
     from qubo_lib_amd import engine
     from qubo_lib_amd.problems.generators import maxcut_gnp, symmetric_int_qubo, symmetric_normal_qubo
     if args.workload == "float":
-        n = 34
+        n = args.float_n
         Q = symmetric_normal_qubo(n, args.seed)
         workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: filter walk over quantized tables, FP64 exact re-walk"
         dtype = "f32"
@@ -179,7 +180,7 @@ def exhaustive(args, ctx, devinfo):
                                                        "walk": ref.info["walk"], "kernel_ms": ref.info["kernel_ms"]},
                       matches_int32_walk=bool(ref.E == E * k and ref.state == state))
     gold = ROOT / "tests" / "golden" / "oracle_goldens.json"
-    key = {"float": f"sym_normal_n34_s{args.seed}", "maxcut": f"maxcut_n40_s{args.seed}"}.get(
+    key = {"float": f"sym_normal_n{n}_s{args.seed}", "maxcut": f"maxcut_n40_s{args.seed}"}.get(
         args.workload, f"sym_int16_n40_s{args.seed}")
     if gold.exists() and key:
         g = json.loads(gold.read_text()).get(key)
@@ -190,7 +191,7 @@ def exhaustive(args, ctx, devinfo):
             parity["matches_oracle"] = bool(g["state"] == state and
                                             (g["E"] == E if isinstance(E, int) else abs(g["E"] - E) < 1e-9 * max(1, abs(E))))
     line = {
-        "metric": {"float": "states/s at n=34 float32 (FP64-exact argmin)",
+        "metric": {"float": f"states/s at n={n} float32 (FP64-exact argmin)",
                    "maxcut": "states/s at n=40 MaxCut G(40, 0.5) (exact, packed pair walk)"}.get(
                        args.workload, "states/s at n=40 int16 (exact)"),
         "value": value,
@@ -213,7 +214,9 @@ def exhaustive(args, ctx, devinfo):
                    "clock_mhz": devinfo["clock_khz"] / 1000.0},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TOP/s",
                      "frac": achieved / peak,
-                     "traffic": measured_traffic({"float": "float", "maxcut": "maxcut"}.get(args.workload, "exhaustive")),
+                     # PMC traffic is committed for the default instances only
+                     "traffic": (None if args.workload == "float" and n != 34 else
+                                 measured_traffic({"float": "float", "maxcut": "maxcut"}.get(args.workload, "exhaustive"))),
                      "ops_per_state": OPS_PER_STATE,
                      "note": "kernel time = HIP events around prep+walk on the library stream" +
                              (f"; walk {r.info['walk']}: peak = packed 16-bit VALU ops (2 per lane per instruction)"
@@ -497,6 +500,7 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[2000.0, 20000.0, 100000.0])
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--float-n", type=int, default=34, help="size of the float workload's instance")
     ap.add_argument("--walk-bits", type=int, default=None)
     ap.add_argument("--inner-bits", type=int, default=None)
     ap.add_argument("--batch-count", type=int, default=4096)

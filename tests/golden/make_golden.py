@@ -38,6 +38,7 @@ def workloads():
         w[f"sym_int16_n40_s{s}"] = ("symmetric_int_qubo(40, seed, -127, 127, int16)", 40, s,
                                     lambda s=s: symmetric_int_qubo(40, s, -127, 127, np.int16))
     w["sym_normal_n34_s0"] = ("symmetric_normal_qubo(34, seed)", 34, 0, lambda: symmetric_normal_qubo(34, 0))
+    w["sym_normal_n40_s0"] = ("symmetric_normal_qubo(40, seed)", 40, 0, lambda: symmetric_normal_qubo(40, 0))
     w["maxcut_n40_s0"] = ("maxcut_gnp(40, 0.5, seed=seed)", 40, 0, lambda: maxcut_gnp(40, 0.5, seed=0))
     w["tsp6_seed42_f32"] = ("TSP.init_random_instance(seed=42, n_cities=6)", 36, 42,
                             lambda: TSP.init_random_instance(seed=42, n_cities=6).build_Q_matrix())
