@@ -166,8 +166,9 @@ def _golden_problem(name):
         n = int(name.split("_n")[1].split("_")[0])
         return symmetric_int_qubo(n, int(name[-1]), -127, 127, np.int16)
     if name.startswith("sym_normal_n"):
-        n = int(name.split("_n")[1].split("_")[0])
-        return symmetric_normal_qubo(n, int(name[-1]))
+        import re
+        n, seed = map(int, re.fullmatch(r"sym_normal_n(\d+)_s(\d+)", name).groups())
+        return symmetric_normal_qubo(n, seed)
     if name == "tsp6_seed42_f32":
         return TSP.init_random_instance(seed=42, n_cities=6).build_Q_matrix()
     if name == "random_n37_seed42_f32":
