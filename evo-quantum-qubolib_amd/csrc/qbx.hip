@@ -58,6 +58,7 @@ struct Layout {
     size_t total_bytes = 0;
     // one part of a single problem's chunks (rank-space shard), or all of them
     uint32_t part_begin = 0, part_count = 0;   // part_count 0: every chunk
+    int32_t flags = 0;                         // qbx_opts_t.flags of the call
     // matrices generated on the device: a small record per problem is staged instead of Q
     // (RandomQUBO: the PCG64 state; TSP: the city distances)
     bool gen = false;
@@ -111,7 +112,11 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     if (!ns) return set_err(QBX_EINVAL, "null argument");
     qbx_opts_t o = {-1, -1, -1, 0};
     if (opts) o = *opts;
-    if (o.flags != 0) return set_err(QBX_EINVAL, "opts.flags must be 0");
+    if (o.flags & ~(QBX_FLAG_FLOAT_INT32_FILTER | QBX_FLAG_FLOAT_PACKED_FILTER))
+        return set_err(QBX_EINVAL, "unknown opts.flags bits 0x%x", o.flags);
+    if ((o.flags & QBX_FLAG_FLOAT_INT32_FILTER) && (o.flags & QBX_FLAG_FLOAT_PACKED_FILTER))
+        return set_err(QBX_EINVAL, "opts.flags: at most one float filter may be forced");
+    lay.flags = o.flags;
     lay.kind = kind;
     lay.count = count;
     lay.esize = kind == KIND_I32 ? sizeof(int32_t) : sizeof(double);
@@ -190,7 +195,7 @@ void stage_descriptors(const Layout& lay, char* h, char* dv) {
         InstDev& I = hinst[i];
         memset(&I, 0, sizeof I);
         I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = lay.kind; I.status = 0; I.tiles = p.tiles;
-        I.tpc = p.tpc; I.chunks = p.chunks;
+        I.tpc = p.tpc; I.chunks = p.chunks; I.flags = plan_flags(p, lay.flags);
         I.expect = lay.part_count ? lay.part_count : p.chunks;
         I.Q = dv + p.off_Q;
         I.W = dv + p.off_W;

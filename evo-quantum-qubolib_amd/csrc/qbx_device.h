@@ -58,7 +58,7 @@ struct InstDev {
                              // QBX_PATH_FLOAT | _FLOAT_PAIR16 (float kind)
     int32_t slack;           // rebased pair walks: PAIR_BOUND - D, the guard's room
     int32_t qshift;          // packed float filter: coarse scale 2^-qshift of the quantized one
-    int32_t pad;
+    int32_t flags;           // qbx_opts_t.flags (QBX_FLAG_FLOAT_INT32_FILTER)
 };
 
 // Table regions of one problem (bytes), shared by the batch and serving layouts.
@@ -215,7 +215,8 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             I.status = (ok && red_i[0] <= 0x7ffffffeLL) ? 0 : QBX_ERANGE;
             // the packed filter wherever a coarse scale leaves its guard room
             int64_t D = 0;
-            const int sh = (I.status == 0) ? qrp_shift(wq_head, dq, n, M, I.L, &D) : -1;
+            const int sh = (I.status == 0 && !(I.flags & QBX_FLAG_FLOAT_INT32_FILTER))
+                               ? qrp_shift(wq_head, dq, n, M, I.L, &D) : -1;
             I.path = sh >= 0 ? QBX_PATH_FLOAT_PAIR16 : QBX_PATH_FLOAT;
             I.qshift = sh >= 0 ? sh : 0;
             I.slack = sh >= 0 ? (int32_t)(PAIR_BOUND - D) : 0;
@@ -512,8 +513,8 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                     const uint64_t lam = lam0 + (uint64_t)lane;
                     const bool active = H >= 6 || lam < (1ull << H);
                     const LaneOutF64 o = walk_lane_qrp<M, OMAX>(Wq, dq, Wpc, W, d, n, L, I.B, active ? lam : lam0,
-                                                                dlp, dld, I.qscale, I.qmargin, I.qshift, I.slack,
-                                                                wb, qc + lane, qmn + lane, 64, nullptr);
+                                                                dlp, dsrc, dld, I.qscale, I.qmargin, I.qshift,
+                                                                I.slack, wb, qc + lane, qmn + lane, 64, nullptr);
                     if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
                 }
             } else {

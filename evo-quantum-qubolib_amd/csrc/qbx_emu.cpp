@@ -158,8 +158,8 @@ int emu_f64(const double* Q, int n, int B, int L, double* E_out, uint64_t* rank_
         for (uint64_t lam = 0; lam < lanes; ++lam) {
             uint32_t fallbacks = 0;
             LaneOutF64 o = walk_lane_qrp<M, OMAX>(Wq.data(), dq.data(), Wpc.data(), W.data(), d.data(), n, L, B,
-                                                  lam, dlpc.data(), dld.data(), scale, qm, sh, slack, hp, buf_c,
-                                                  buf_m, 1, &fallbacks);
+                                                  lam, dlpc.data(), dlq.data(), dld.data(), scale, qm, sh, slack, hp,
+                                                  buf_c, buf_m, 1, &fallbacks);
             if (o.E < pE || (o.E == pE && o.rank < pR)) { pE = o.E; pR = o.rank; }
             ++emu_stat_qrp_lanes;
             emu_stat_qrp_int32_groups += fallbacks;

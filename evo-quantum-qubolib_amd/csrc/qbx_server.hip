@@ -181,7 +181,7 @@ int admit(qbx_server* S, std::vector<Req*>& batch) {
         InstDev& I = hi[j];
         memset(&I, 0, sizeof I);
         I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = r->kind; I.status = 0;
-        I.tiles = p.tiles; I.tpc = p.tpc; I.chunks = p.chunks; I.expect = p.chunks;
+        I.tiles = p.tiles; I.tpc = p.tpc; I.chunks = p.chunks; I.expect = p.chunks; I.flags = plan_flags(p, 0);
         I.Q = dv + off_q + j * qstride;
         I.W = base + S->L.off_W;
         I.d = base + S->L.off_d;

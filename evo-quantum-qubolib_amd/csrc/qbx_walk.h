@@ -1373,9 +1373,10 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
 //     added and took away the same coarse row): at most 4 x 1/2 = 2,
 //   * 1/2 per trie constant on y's path (at most M of them),
 // so |R(y) / 2^sh - R_c(y)| <= M (1/2 + 2) + M / 2 = 3M <= QRP_ERR.  A block whose coarse
-// minimum is m (unbiased) therefore holds no state with Eq < E0 + (m - QRP_ERR) 2^sh (the
-// filter tests this bound against the threshold of walk_lane_q32) and holds a state with
-// Eq <= E0 + (m + QRP_ERR) 2^sh (what the lane minimum takes).
+// minimum is m (unbiased) therefore holds no state with Eq < E0 + (m - QRP_ERR) 2^sh.  A
+// block for which that bound passes the threshold of walk_lane_q32 gets the int32 filter's
+// own test (qrp_refine: its exact int32 minimum, from exact fields and constants), and only
+// blocks passing that are recorded: the coarse walk is a pre-filter of the int32 filter.
 //
 // Range.  Every coarse energy a packed group materialises is a state's, and
 //     |R_c(y)| <= sum_k |fc_k(group start)| + D_c,
@@ -1450,32 +1451,44 @@ QBX_HD int qrp_shift(const int32_t* Wq, const int32_t* dq, int n, int M, int L, 
     return -1;
 }
 
-// Candidate record of packed block c from the minima of its halves (rare path).
+// Rare path: the halves of packed block c whose coarse bound passes (m[h] <= t[h]) get
+// the int32 filter's own test -- their exact minimum on the quantized tables (fields f of
+// the low block, the high block's adding row M; trie constants dlq) against the threshold --
+// and the block is recorded only if one passes, with that exact minimum.  So the coarse
+// walk is a pre-filter that never drops a block the int32 filter would keep (its error
+// bound), and what is recorded and re-walked in FP64 is what the int32 filter records.
 template <int M, int OMAX>
-QBX_HD void qrp_record(const LaneRP<M, OMAX>& s, uint32_t c, const int32_t* m, const int32_t* t, int32_t bias,
-                       int sh, QLane& q, QBuf& b) {
+QBX_HD void qrp_refine(const LaneRP<M, OMAX>& s, uint32_t c, const int32_t* f, const int32_t* m, const int32_t* t,
+                       const int32_t* __restrict__ Wq, const int32_t* __restrict__ dlq, QLane& q, QBuf& b) {
+    const int32_t* WM = Wq + M * LDW;
     uint32_t mask = 0;
-    int64_t lb = 0x7fffffffffffffffLL;
+    int32_t best = 0x7fffffff;
+#pragma unroll 1
     for (int h = 0; h < 2; ++h) {
         if (m[h] > t[h]) continue;
-        mask |= 1u << h;
-        const int64_t base = (int64_t)s.E0 + (h ? (int64_t)s.up : 0);
-        const int64_t lo = base + ((int64_t)(m[h] - bias - QRP_ERR) << sh);
-        const int32_t ub = clamp_i32(base + ((int64_t)(m[h] - bias + QRP_ERR) << sh));
-        lb = lo < lb ? lo : lb;
-        q.laneMin = ub < q.laneMin ? ub : q.laneMin;
+        int32_t fh[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) fh[k] = (int32_t)((uint32_t)f[k] + (h ? (uint32_t)WM[k] : 0u));
+        const int32_t base = (int32_t)((uint32_t)s.E0 + (h ? (uint32_t)s.up : 0u));
+        const int32_t mn = block_min_i32<M>(base, fh, dlq);
+        if (mn <= q.thr) {
+            mask |= 1u << h;
+            best = mn < best ? mn : best;
+        }
     }
+    if (!mask) return;
+    q.laneMin = best < q.laneMin ? best : q.laneMin;
     q_update_thr(q);
     q.thr = q.thr < QRP_THR_MAX ? q.thr : QRP_THR_MAX;
     b.c[b.count * b.stride] = (c << 2) | mask;
-    b.m[b.count * b.stride] = clamp_i32(lb);
+    b.m[b.count * b.stride] = best;
     ++b.count;
 }
 
 template <int M, int OMAX, int J>
 QBX_HD bool group_step_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ Wq,
-                           const uint32_t* __restrict__ Wpc, const uint32_t* dl, uint32_t c0, uint32_t nblk,
-                           int sh, QLane& q, QBuf& b) {
+                           const uint32_t* __restrict__ Wpc, const uint32_t* dl, const int32_t* __restrict__ dlq,
+                           uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b) {
     const uint32_t c = c0 + (uint32_t)J;
     if (J > 0) {
         if (c >= nblk) return false;
@@ -1495,15 +1508,34 @@ QBX_HD bool group_step_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __res
     m[1] = (int32_t)(mp >> 16);
     t[0] = ((q.thr - s.E0) >> sh) + (PAIR_BIAS + QRP_ERR);
     t[1] = ((q.thr - (s.E0 + s.up)) >> sh) + (PAIR_BIAS + QRP_ERR);
-    if (__builtin_expect(m[0] <= t[0] || m[1] <= t[1], 0)) qrp_record<M, OMAX>(s, c, m, t, PAIR_BIAS, sh, q, b);
+    if (__builtin_expect(m[0] <= t[0] || m[1] <= t[1], 0)) {
+        // exact fields of block c: the group start's (s.f) plus the rows of the group bits
+        // flipped since, gray(J) (bit GROUP_BITS - 1 with the sign of bit GROUP_BITS of c0,
+        // lower bits upwards: their values at the start are 0)
+        int32_t f[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) f[k] = s.f[k];
+        constexpr unsigned G = (unsigned)J ^ ((unsigned)J >> 1);
+        const bool top_up = ((c0 >> GROUP_BITS) & 1u) == 0u;
+#pragma unroll
+        for (int i = 0; i < GROUP_BITS; ++i) {
+            if (!((G >> i) & 1u)) continue;
+            const int32_t* WJ = Wq + (M + 1 + i) * LDW;
+            const bool up = (i + 1 < GROUP_BITS) || top_up;
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+                f[k] = (int32_t)(up ? (uint32_t)f[k] + (uint32_t)WJ[k] : (uint32_t)f[k] - (uint32_t)WJ[k]);
+        }
+        qrp_refine<M, OMAX>(s, c, f, m, t, Wq, dlq, q, b);
+    }
     return true;
 }
 
 template <int M, int OMAX, int... Js>
 QBX_HD void group_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ Wq,
-                      const uint32_t* __restrict__ Wpc, const uint32_t* dl, uint32_t c0, uint32_t nblk, int sh,
-                      QLane& q, QBuf& b, std::integer_sequence<int, Js...>) {
-    (void)(group_step_qrp<M, OMAX, Js>(s, F, Wq, Wpc, dl, c0, nblk, sh, q, b) && ...);
+                      const uint32_t* __restrict__ Wpc, const uint32_t* dl, const int32_t* __restrict__ dlq,
+                      uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b, std::integer_sequence<int, Js...>) {
+    (void)(group_step_qrp<M, OMAX, Js>(s, F, Wq, Wpc, dl, dlq, c0, nblk, sh, q, b) && ...);
 }
 
 // The group at c0 with coarse values in int32 (a wave whose guard failed): the fine state
@@ -1511,7 +1543,8 @@ QBX_HD void group_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict
 // fields rounded from the fine ones.
 template <int M, int OMAX>
 QBX_HD void group_qrp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ Wq, const uint32_t* dl,
-                            uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b) {
+                            const int32_t* __restrict__ dlq, uint32_t c0, uint32_t nblk, int sh, QLane& q,
+                            QBuf& b) {
     const int32_t* WM = Wq + M * LDW;
     const uint32_t cend = (c0 + (1u << GROUP_BITS) < nblk) ? c0 + (1u << GROUP_BITS) : nblk;
 #pragma unroll 1
@@ -1535,7 +1568,7 @@ QBX_HD void group_qrp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ Wq, 
             m[h] = mn;
             t[h] = ((q.thr - base) >> sh) + QRP_ERR;
         }
-        if (m[0] <= t[0] || m[1] <= t[1]) qrp_record<M, OMAX>(s, c, m, t, 0, sh, q, b);
+        if (m[0] <= t[0] || m[1] <= t[1]) qrp_refine<M, OMAX>(s, c, s.f, m, t, Wq, dlq, q, b);
     }
 }
 
@@ -1546,9 +1579,10 @@ template <int M, int OMAX, class Bound>
 QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
                                 const uint32_t* __restrict__ Wpc, const double* __restrict__ W,
                                 const double* __restrict__ d, int n, int L, int B, uint64_t lam,
-                                const uint32_t* dlpc, const double* __restrict__ dld, double scale, double qm,
-                                int sh, int32_t slack, Bound& src, uint32_t* buf_c, int32_t* buf_m,
-                                int buf_stride, uint32_t* groups_int32) {
+                                const uint32_t* dlpc, const int32_t* __restrict__ dlq,
+                                const double* __restrict__ dld, double scale, double qm, int sh, int32_t slack,
+                                Bound& src, uint32_t* buf_c, int32_t* buf_m, int buf_stride,
+                                uint32_t* groups_int32) {
     static_assert(OMAX > GROUP_BITS, "OMAX must cover the unrolled transition bits");
     const int O = L - M;
     LaneRP<M, OMAX> s;
@@ -1589,7 +1623,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
 #pragma unroll
             for (int k = 0; k < M; ++k)
                 F[k] = (uint32_t)fc[k] + ((uint32_t)rnd_shr((int32_t)((uint32_t)s.f[k] + (uint32_t)WM[k]), sh) << 16);
-            group_qrp<M, OMAX>(s, F, Wq, Wpc, dlpc, c0, nblk, sh, q, b,
+            group_qrp<M, OMAX>(s, F, Wq, Wpc, dlpc, dlq, c0, nblk, sh, q, b,
                                std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
             // the fine fields follow the group's net change: a full group flips group bit
             // GROUP_BITS - 1 once (every lower bit an even number of times); after a partial
@@ -1601,7 +1635,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
                     s.f[k] = (int32_t)(up ? (uint32_t)s.f[k] + (uint32_t)W3[k] : (uint32_t)s.f[k] - (uint32_t)W3[k]);
             }
         } else {
-            group_qrp_int32<M, OMAX>(s, Wq, dlpc, c0, nblk, sh, q, b);
+            group_qrp_int32<M, OMAX>(s, Wq, dlpc, dlq, c0, nblk, sh, q, b);
             if (groups_int32) ++*groups_int32;
         }
         if (b.count > QCAP - (1 << GROUP_BITS)) {   // room for one more group of records

@@ -276,10 +276,17 @@ def _check(lib, rc: int):
     raise NativeError(rc, msg)
 
 
-def _opts(tie_B=None, walk_bits=None, inner_bits=None) -> Opts:
+# float problems: None = by size (packed from n = 22 in batches, n = 28 alone), or forced
+FLOAT_FILTERS = {None: 0, "int32": 1, "packed": 2}   # QBX_FLAG_FLOAT_INT32_FILTER / _PACKED_FILTER
+
+
+def _opts(tie_B=None, walk_bits=None, inner_bits=None, float_filter=None) -> Opts:
+    """float_filter: None (by problem size), "packed" (wherever it can run) or "int32"."""
+    if float_filter not in FLOAT_FILTERS:
+        raise ValueError(f"float_filter must be one of {sorted(k for k in FLOAT_FILTERS if k)}: {float_filter!r}")
     return Opts(-1 if tie_B is None else int(tie_B),
                 -1 if walk_bits is None else int(walk_bits),
-                -1 if inner_bits is None else int(inner_bits), 0)
+                -1 if inner_bits is None else int(inner_bits), FLOAT_FILTERS[float_filter])
 
 
 class Context:

@@ -110,19 +110,20 @@ def _finish(p: Prepared, r: _native.Result) -> RawResult:
     return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=p.n, tie_B=r.tie_B, kind=p.kind, info=info)
 
 
-def solve_matrix(Q, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None) -> RawResult:
-    """Exact minimum of x^T Q x and its argmin under the reference's tie rule."""
+def solve_matrix(Q, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None) -> RawResult:
+    """Exact minimum of x^T Q x and its argmin under the reference's tie rule.
+    float_filter (float problems): None (by size), "packed" or "int32"; the answer is the same."""
     p = Q if isinstance(Q, Prepared) else prepare(Q)
     if tie_B is not None and not (0 <= int(tie_B) <= p.n):
         raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
     ctx = _native.get_context(device)
-    kw = dict(tie_B=tie_B, walk_bits=walk_bits, inner_bits=inner_bits)
+    kw = dict(tie_B=tie_B, walk_bits=walk_bits, inner_bits=inner_bits, float_filter=float_filter)
     r = ctx.solve_i32(p.matrix, **kw) if p.kind == "i32" else ctx.solve_f64(p.matrix, **kw)
     return _finish(p, r)
 
 
 def solve_part(Q, part: int, parts: int, device: int = 0, tie_B=None, walk_bits=None,
-               inner_bits=None) -> RawResult:
+               inner_bits=None, float_filter=None) -> RawResult:
     """The (E, rank) minimum over one of `parts` equal shares of the solve's state space (one
     GPU's share of a solve spread over `parts` GPUs); merge_parts of all shares is
     solve_matrix's answer.  The share boundaries follow the solve's tiling, so every share
@@ -132,7 +133,7 @@ def solve_part(Q, part: int, parts: int, device: int = 0, tie_B=None, walk_bits=
         raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
     ctx = _native.get_context(device)
     r = ctx.solve_part(p.matrix, part, parts, kind=p.kind, tie_B=tie_B, walk_bits=walk_bits,
-                       inner_bits=inner_bits)
+                       inner_bits=inner_bits, float_filter=float_filter)
     return _finish(p, r)
 
 
@@ -163,7 +164,7 @@ def raw_from_native(r: _native.Result, integer: bool) -> RawResult:
     return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=int(r.n), tie_B=r.tie_B, kind=kind, info=info)
 
 
-def solve_matrices(Qs, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None):
+def solve_matrices(Qs, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None):
     """Many problems, one persistent launch per arithmetic path.  Returns RawResults in
     input order.  `tie_B` may be a single value (None = min(N, 20) per problem)."""
     preps = [q if isinstance(q, Prepared) else prepare(q) for q in Qs]
@@ -174,7 +175,7 @@ def solve_matrices(Qs, device: int = 0, tie_B=None, walk_bits=None, inner_bits=N
         if not idx:
             continue
         res = ctx.batch([preps[i].matrix for i in idx], kind=kind, tie_B=tie_B, walk_bits=walk_bits,
-                        inner_bits=inner_bits)
+                        inner_bits=inner_bits, float_filter=float_filter)
         for i, r in zip(idx, res):
             if r.status != 0:
                 raise OverflowError(f"problem {i}: energy bound outside the {kind} path")

@@ -43,8 +43,17 @@ typedef struct {
     int32_t tie_B;        /* reference suffix width B for the tie rule; -1 = min(n, 20) */
     int32_t walk_bits;    /* bits walked by one lane (L); -1 = tuned default */
     int32_t inner_bits;   /* unrolled inner block bits (M: 1, 4, 6 or 7); -1 = 7 */
-    int32_t flags;        /* reserved, must be 0 */
+    int32_t flags;        /* QBX_FLAG_* bits, 0 = defaults */
 } qbx_opts_t;
+
+/* opts.flags, float problems (same answers either way): by default the packed 16-bit filter
+ * runs where it can for problems of n >= 22 in batches (QBX_MODE_BATCH, and serving) and
+ * n >= 28 solved alone, the int32 filter below (measured: the packed filter's set-up does
+ * not pay for smaller problems); QBX_FLAG_FLOAT_INT32_FILTER takes the int32 filter always
+ * (result path QBX_PATH_FLOAT), QBX_FLAG_FLOAT_PACKED_FILTER the packed one wherever it can
+ * run (QBX_PATH_FLOAT_PAIR16).  At most one of the two. */
+#define QBX_FLAG_FLOAT_INT32_FILTER 1
+#define QBX_FLAG_FLOAT_PACKED_FILTER 2
 
 typedef struct {
     int64_t E_int;        /* exact minimum energy (integer solves) */

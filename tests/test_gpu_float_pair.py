@@ -38,7 +38,7 @@ def test_packed_float_filter_vs_oracle(gpu_ctx, inner):
         E_ref, s_ref = oracle.brute_f64(Q, B)
         for L in sorted({inner + 2, min(n, inner + 5), min(n, inner + omax)}):
             flush("float pair", inner, trial, n, B, L)
-            got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=inner)
+            got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=inner, float_filter="packed")
             assert got.info["walk"] == "float_pair16", (inner, trial, n, L, got.info["walk"])
             assert got.state == s_ref and close(got.E, E_ref, Q), (inner, trial, n, B, L)
 
@@ -55,7 +55,7 @@ def test_packed_float_filter_matches_int_walk_large(gpu_ctx):
             Q = random_instance(rng, n, kind)
         for B in (None, 0):
             flush("float pair large", n, kind, B)
-            a = engine.solve_matrix(Q.astype(np.float64) / 4, tie_B=B)
+            a = engine.solve_matrix(Q.astype(np.float64) / 4, tie_B=B, float_filter="packed")
             b = engine.solve_matrix(Q, tie_B=B)
             assert a.info["walk"] == "float_pair16", a.info["walk"]
             assert (a.E * 4, a.state) == (b.E, b.state), (n, kind, B)
@@ -64,12 +64,32 @@ def test_packed_float_filter_matches_int_walk_large(gpu_ctx):
 def test_packed_float_filter_in_batches_and_shares(gpu_ctx):
     rng = np.random.default_rng(972)
     Qs = [rng.standard_normal((n, n)) for n in rng.integers(10, 21, 30)]
-    got = engine.solve_matrices(Qs)
+    got = engine.solve_matrices(Qs, float_filter="packed")
     for i, (Q, r) in enumerate(zip(Qs, got)):
         E_ref, s_ref = oracle.brute_f64(Q)
         assert r.state == s_ref and close(r.E, E_ref, Q), i
+        packable = r.info["walk_bits"] > r.info["inner_bits"] + 1     # a pair bit and a walked bit
+        assert r.info["walk"] == ("float_pair16" if packable else "float"), i
     Q = rng.standard_normal((26, 26))
-    full = engine.solve_matrix(Q, walk_bits=14)
+    full = engine.solve_matrix(Q, walk_bits=14, float_filter="packed")
     assert full.info["walk"] == "float_pair16"
-    merged = engine.merge_parts([engine.solve_part(Q, k, 8, walk_bits=14) for k in range(8)])
+    merged = engine.merge_parts([engine.solve_part(Q, k, 8, walk_bits=14, float_filter="packed")
+                                 for k in range(8)])
     assert (merged.E, merged.state) == (full.E, full.state)
+
+
+def test_float_filter_choice(gpu_ctx):
+    """By default the packed filter runs from n = 28 alone and n = 22 in batches; either
+    filter can be forced; all give the same answer."""
+    rng = np.random.default_rng(973)
+    small, large = rng.standard_normal((24, 24)), rng.standard_normal((30, 30))
+    assert engine.solve_matrix(small).info["walk"] == "float"
+    assert engine.solve_matrix(large).info["walk"] == "float_pair16"
+    batch = engine.solve_matrices([small, rng.standard_normal((18, 18))])
+    assert [r.info["walk"] for r in batch] == ["float_pair16", "float"]
+    for Q in (small, large):
+        runs = [engine.solve_matrix(Q, float_filter=f) for f in (None, "packed", "int32")]
+        assert [r.info["walk"] for r in runs[1:]] == ["float_pair16", "float"]
+        assert len({(r.E, r.state) for r in runs}) == 1
+    with pytest.raises(ValueError):
+        engine.solve_matrix(small, float_filter="fp16")

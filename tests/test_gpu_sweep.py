@@ -94,7 +94,7 @@ def test_packed_float_filter_sweep(gpu_ctx):
         L = int(rng.integers(M, min(n, M + OMAX) + 1))
         B = int(rng.integers(0, n + 1)) if trial % 3 else None
         E, s = oracle.brute_f64(Q, B)
-        got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=M)
+        got = engine.solve_matrix(Q, tie_B=B, walk_bits=L, inner_bits=M, float_filter="packed")
         assert got.state == s and abs(got.E - E) <= 1e-13 * np.abs(Q).sum() + 1e-300, \
             ("float sweep", trial, n, kind, B, L, M, got.info["walk"])
         seen[got.info["walk"]] = seen.get(got.info["walk"], 0) + 1
