@@ -1392,9 +1392,6 @@ constexpr int32_t QRP_THR_MAX = 0x20100000;   // above every quantized energy (<
 
 // v / 2^sh rounded to nearest, halves up (|v| <= 2^30, 0 <= sh <= 30)
 QBX_HD int32_t rnd_shr(int32_t v, int sh) { return sh == 0 ? v : ((v + (1 << (sh - 1))) >> sh); }
-QBX_HD int32_t clamp_i32(int64_t v) {
-    return v > 0x7fffffffLL ? 0x7fffffff : (v < -0x80000000LL ? (int32_t)0x80000000 : (int32_t)v);
-}
 
 // D_c of the guard for shift sh, or -1 when a coarse table value could leave 15 bits.
 QBX_HD int64_t qrp_guard_D(const int32_t* Wq, int M, int L, int sh) {
