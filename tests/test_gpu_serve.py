@@ -130,3 +130,22 @@ def test_submit_many_matches_individual_solves(gpu_ctx):
     for q, i in zip(Qs, ids):
         alone = engine.solve_matrix(q)
         assert got[i]["status"] == 0 and (got[i]["E"], got[i]["state"]) == (alone.E, alone.state), q.shape
+
+
+def test_server_packed_float_filter_requests(gpu_ctx):
+    """Float requests of n = 22 .. 26 run on the packed float filter in the server (its
+    batch rule); answers equal the same problems solved alone (the int32 filter below
+    n = 28) and, up to n = 24, the FP64 brute force."""
+    rng = np.random.default_rng(33)
+    Qs = [rng.standard_normal((n, n)) for n in (22, 23, 24, 26, 22, 25)]
+    with Server(max_n=40, max_inflight=16) as srv:
+        ids = [srv.submit(q) for q in Qs]
+        got = drain(srv, len(ids))
+    for q, i in zip(Qs, ids):
+        alone = engine.solve_matrix(q)
+        r = got[i]
+        assert alone.info["walk"] == "float"
+        assert r["status"] == 0 and (r["E"], r["state"]) == (alone.E, alone.state), q.shape
+        if q.shape[0] <= 24:
+            E, s = oracle.brute_f64(q)
+            assert r["state"] == s and abs(r["E"] - E) <= 1e-13 * np.abs(q).sum()
