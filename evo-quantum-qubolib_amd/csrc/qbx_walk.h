@@ -1162,6 +1162,9 @@ QBX_HD void block_exact_f64(const double* __restrict__ W, const double* __restri
 // in FP64 inside the unrolled group.
 constexpr int QCAP = 32;
 constexpr uint32_t QSYNC = 8;   // groups between exchanges of the shared bound
+// cap on the packed float filter's thresholds: above every quantized energy (< 2^29), so
+// that threshold - base fits int32
+constexpr int32_t QRP_THR_MAX = 0x20100000;
 
 struct QBuf {
     uint32_t* c;
@@ -1193,6 +1196,11 @@ struct QCtx {    // what a flush needs to rebuild and walk a block exactly
     const double* dld;
     int n, L, B;
     uint64_t lam;    // the lane; its prefix, rank bits and block-key mask are derived per flush
+    // packed float filter: the quantized tables, for the int32 filter's exact test of a
+    // record before its FP64 walk
+    const int32_t* Wq = nullptr;
+    const int32_t* dq = nullptr;
+    const int32_t* dlq = nullptr;
     QBX_HD_MEMBER void set(const double* W_, const double* d_, const double* dld_, int n_, int L_, int B_,
                            uint64_t lam_) {
         W = W_; d = d_; dld = dld_; n = n_; L = L_; B = B_; lam = lam_;
@@ -1202,9 +1210,12 @@ struct QCtx {    // what a flush needs to rebuild and walk a block exactly
 // Drop the records the filter no longer passes; with `walk`, walk the rest in FP64 and
 // empty the buffer.  PACKED records (the packed float filter) name a packed block c and
 // the halves h that passed, as (c << 2) | mask; half h is block (c << 1) | (h ^ (c & 1))
-// of the int32 walk's numbering.
+// of the int32 walk's numbering.  Their minimum m is a coarse lower bound, so before its
+// FP64 walk each half gets the int32 filter's own test: its exact int32 minimum on the
+// quantized tables, rebuilt from scratch, against the threshold.  What is walked in FP64
+// is then what the int32 filter would walk.
 template <int M, bool PACKED = false>
-QBX_HD void q_compact(QBuf& b, const QLane& q, const QCtx<M>& x, bool walk, double& bestE, uint64_t& bestR) {
+QBX_HD void q_compact(QBuf& b, QLane& q, const QCtx<M>& x, bool walk, double& bestE, uint64_t& bestR) {
     const int O = x.L - M;
     const uint64_t p = lane_prefix(x.lam, x.L, x.B);
     const uint64_t laneRank = shl64(x.lam, x.L);
@@ -1221,9 +1232,21 @@ QBX_HD void q_compact(QBuf& b, const QLane& q, const QCtx<M>& x, bool walk, doub
                 const uint32_t cp = c >> 2;
                 for (uint32_t h = 0; h < 2u; ++h) {
                     if (!((c >> h) & 1u)) continue;
+                    const uint32_t cf = (cp << 1) | (h ^ (cp & 1u));
+                    int32_t E0, f[M];
+                    block_state_i32<M>(x.Wq, x.dq, x.n, (p << O) | (uint64_t)(cf ^ (cf >> 1)), E0, f);
+                    const int32_t mn = block_min_i32<M>(E0, f, x.dlq);
+                    // an exact minimum is a state's quantized energy: the lane minimum may
+                    // take it (the coarse records only gave it upper bounds)
+                    if (mn < q.laneMin) {
+                        q.laneMin = mn;
+                        q_update_thr(q);
+                        q.thr = q.thr < QRP_THR_MAX ? q.thr : QRP_THR_MAX;
+                    }
+                    if (mn > q.thr) continue;
                     QBX_STAT(++emu_stat_rewalks);
-                    block_exact_f64<M>(x.W, x.d, x.n, p, O, (cp << 1) | (h ^ (cp & 1u)), laneRank, SBk, pmaskS,
-                                       x.B, parp, x.dld, bestE, bestR);
+                    block_exact_f64<M>(x.W, x.d, x.n, p, O, cf, laneRank, SBk, pmaskS, x.B, parp, x.dld, bestE,
+                                       bestR);
                 }
             } else {
                 QBX_STAT(++emu_stat_rewalks);
@@ -1373,10 +1396,12 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
 //     added and took away the same coarse row): at most 4 x 1/2 = 2,
 //   * 1/2 per trie constant on y's path (at most M of them),
 // so |R(y) / 2^sh - R_c(y)| <= M (1/2 + 2) + M / 2 = 3M <= QRP_ERR.  A block whose coarse
-// minimum is m (unbiased) therefore holds no state with Eq < E0 + (m - QRP_ERR) 2^sh.  A
-// block for which that bound passes the threshold of walk_lane_q32 gets the int32 filter's
-// own test (qrp_refine: its exact int32 minimum, from exact fields and constants), and only
-// blocks passing that are recorded: the coarse walk is a pre-filter of the int32 filter.
+// minimum is m (unbiased) therefore holds no state with Eq < E0 + (m - QRP_ERR) 2^sh (the
+// record's bound, tested against the threshold of walk_lane_q32) and holds a state with
+// Eq <= E0 + (m + QRP_ERR) 2^sh (what the lane minimum takes).  A record that still passes
+// when it is due for its FP64 walk first gets the int32 filter's own test, its exact
+// minimum rebuilt from scratch (q_compact): the coarse walk is a pre-filter of the int32
+// filter, and nothing is walked in FP64 that the int32 filter would not walk.
 //
 // Range.  Every coarse energy a packed group materialises is a state's, and
 //     |R_c(y)| <= sum_k |fc_k(group start)| + D_c,
@@ -1388,7 +1413,6 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
 // |Eq| < 2^29 (q_scale), and thresholds are capped at QRP_THR_MAX, so threshold - base
 // fits int32.
 constexpr int32_t QRP_ERR = 21;
-constexpr int32_t QRP_THR_MAX = 0x20100000;   // above every quantized energy (< 2^29)
 
 // v / 2^sh rounded to nearest, halves up (|v| <= 2^30, 0 <= sh <= 30)
 QBX_HD int32_t rnd_shr(int32_t v, int sh) { return sh == 0 ? v : ((v + (1 << (sh - 1))) >> sh); }
@@ -1448,44 +1472,39 @@ QBX_HD int qrp_shift(const int32_t* Wq, const int32_t* dq, int n, int M, int L, 
     return -1;
 }
 
-// Rare path: the halves of packed block c whose coarse bound passes (m[h] <= t[h]) get
-// the int32 filter's own test -- their exact minimum on the quantized tables (fields f of
-// the low block, the high block's adding row M; trie constants dlq) against the threshold --
-// and the block is recorded only if one passes, with that exact minimum.  So the coarse
-// walk is a pre-filter that never drops a block the int32 filter would keep (its error
-// bound), and what is recorded and re-walked in FP64 is what the int32 filter records.
+// Candidate record of packed block c from the coarse minima of its halves (rare path): the
+// halves whose coarse bound passes, their least lower bound E0_h + (m_h - bias - QRP_ERR)
+// 2^sh as the record's minimum, and the lane minimum from the upper bound
+// E0_h + (m_h - bias + QRP_ERR) 2^sh of the state realising m_h.
+QBX_HD int32_t clamp_i32(int64_t v) {
+    return v > 0x7fffffffLL ? 0x7fffffff : (v < -0x80000000LL ? (int32_t)0x80000000 : (int32_t)v);
+}
+
 template <int M, int OMAX>
-QBX_HD void qrp_refine(const LaneRP<M, OMAX>& s, uint32_t c, const int32_t* f, const int32_t* m, const int32_t* t,
-                       const int32_t* __restrict__ Wq, const int32_t* __restrict__ dlq, QLane& q, QBuf& b) {
-    const int32_t* WM = Wq + M * LDW;
+QBX_HD void qrp_record(const LaneRP<M, OMAX>& s, uint32_t c, const int32_t* m, const int32_t* t, int32_t bias,
+                       int sh, QLane& q, QBuf& b) {
     uint32_t mask = 0;
-    int32_t best = 0x7fffffff;
-#pragma unroll 1
+    int64_t lb = 0x7fffffffffffffffLL;
     for (int h = 0; h < 2; ++h) {
         if (m[h] > t[h]) continue;
-        int32_t fh[M];
-#pragma unroll
-        for (int k = 0; k < M; ++k) fh[k] = (int32_t)((uint32_t)f[k] + (h ? (uint32_t)WM[k] : 0u));
-        const int32_t base = (int32_t)((uint32_t)s.E0 + (h ? (uint32_t)s.up : 0u));
-        const int32_t mn = block_min_i32<M>(base, fh, dlq);
-        if (mn <= q.thr) {
-            mask |= 1u << h;
-            best = mn < best ? mn : best;
-        }
+        mask |= 1u << h;
+        const int64_t base = (int64_t)s.E0 + (h ? (int64_t)s.up : 0);
+        const int64_t lo = base + ((int64_t)(m[h] - bias - QRP_ERR) << sh);
+        const int32_t ub = clamp_i32(base + ((int64_t)(m[h] - bias + QRP_ERR) << sh));
+        lb = lo < lb ? lo : lb;
+        q.laneMin = ub < q.laneMin ? ub : q.laneMin;
     }
-    if (!mask) return;
-    q.laneMin = best < q.laneMin ? best : q.laneMin;
     q_update_thr(q);
     q.thr = q.thr < QRP_THR_MAX ? q.thr : QRP_THR_MAX;
     b.c[b.count * b.stride] = (c << 2) | mask;
-    b.m[b.count * b.stride] = best;
+    b.m[b.count * b.stride] = clamp_i32(lb);
     ++b.count;
 }
 
 template <int M, int OMAX, int J>
 QBX_HD bool group_step_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ Wq,
-                           const uint32_t* __restrict__ Wpc, const uint32_t* dl, const int32_t* __restrict__ dlq,
-                           uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b) {
+                           const uint32_t* __restrict__ Wpc, const uint32_t* dl, uint32_t c0, uint32_t nblk,
+                           int sh, QLane& q, QBuf& b) {
     const uint32_t c = c0 + (uint32_t)J;
     if (J > 0) {
         if (c >= nblk) return false;
@@ -1505,34 +1524,15 @@ QBX_HD bool group_step_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __res
     m[1] = (int32_t)(mp >> 16);
     t[0] = ((q.thr - s.E0) >> sh) + (PAIR_BIAS + QRP_ERR);
     t[1] = ((q.thr - (s.E0 + s.up)) >> sh) + (PAIR_BIAS + QRP_ERR);
-    if (__builtin_expect(m[0] <= t[0] || m[1] <= t[1], 0)) {
-        // exact fields of block c: the group start's (s.f) plus the rows of the group bits
-        // flipped since, gray(J) (bit GROUP_BITS - 1 with the sign of bit GROUP_BITS of c0,
-        // lower bits upwards: their values at the start are 0)
-        int32_t f[M];
-#pragma unroll
-        for (int k = 0; k < M; ++k) f[k] = s.f[k];
-        constexpr unsigned G = (unsigned)J ^ ((unsigned)J >> 1);
-        const bool top_up = ((c0 >> GROUP_BITS) & 1u) == 0u;
-#pragma unroll
-        for (int i = 0; i < GROUP_BITS; ++i) {
-            if (!((G >> i) & 1u)) continue;
-            const int32_t* WJ = Wq + (M + 1 + i) * LDW;
-            const bool up = (i + 1 < GROUP_BITS) || top_up;
-#pragma unroll
-            for (int k = 0; k < M; ++k)
-                f[k] = (int32_t)(up ? (uint32_t)f[k] + (uint32_t)WJ[k] : (uint32_t)f[k] - (uint32_t)WJ[k]);
-        }
-        qrp_refine<M, OMAX>(s, c, f, m, t, Wq, dlq, q, b);
-    }
+    if (__builtin_expect(m[0] <= t[0] || m[1] <= t[1], 0)) qrp_record<M, OMAX>(s, c, m, t, PAIR_BIAS, sh, q, b);
     return true;
 }
 
 template <int M, int OMAX, int... Js>
 QBX_HD void group_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict__ Wq,
-                      const uint32_t* __restrict__ Wpc, const uint32_t* dl, const int32_t* __restrict__ dlq,
-                      uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b, std::integer_sequence<int, Js...>) {
-    (void)(group_step_qrp<M, OMAX, Js>(s, F, Wq, Wpc, dl, dlq, c0, nblk, sh, q, b) && ...);
+                      const uint32_t* __restrict__ Wpc, const uint32_t* dl, uint32_t c0, uint32_t nblk, int sh,
+                      QLane& q, QBuf& b, std::integer_sequence<int, Js...>) {
+    (void)(group_step_qrp<M, OMAX, Js>(s, F, Wq, Wpc, dl, c0, nblk, sh, q, b) && ...);
 }
 
 // The group at c0 with coarse values in int32 (a wave whose guard failed): the fine state
@@ -1540,8 +1540,7 @@ QBX_HD void group_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __restrict
 // fields rounded from the fine ones.
 template <int M, int OMAX>
 QBX_HD void group_qrp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ Wq, const uint32_t* dl,
-                            const int32_t* __restrict__ dlq, uint32_t c0, uint32_t nblk, int sh, QLane& q,
-                            QBuf& b) {
+                            uint32_t c0, uint32_t nblk, int sh, QLane& q, QBuf& b) {
     const int32_t* WM = Wq + M * LDW;
     const uint32_t cend = (c0 + (1u << GROUP_BITS) < nblk) ? c0 + (1u << GROUP_BITS) : nblk;
 #pragma unroll 1
@@ -1565,7 +1564,7 @@ QBX_HD void group_qrp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ Wq, 
             m[h] = mn;
             t[h] = ((q.thr - base) >> sh) + QRP_ERR;
         }
-        if (m[0] <= t[0] || m[1] <= t[1]) qrp_refine<M, OMAX>(s, c, s.f, m, t, Wq, dlq, q, b);
+        if (m[0] <= t[0] || m[1] <= t[1]) qrp_record<M, OMAX>(s, c, m, t, 0, sh, q, b);
     }
 }
 
@@ -1587,6 +1586,9 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
 
     QCtx<M> x;
     x.set(W, d, dld, n, L, B, lam);
+    x.Wq = Wq;
+    x.dq = dq;
+    x.dlq = dlq;
 
     double bound = src.refresh();
     double bestE = __builtin_inf();
@@ -1620,7 +1622,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
 #pragma unroll
             for (int k = 0; k < M; ++k)
                 F[k] = (uint32_t)fc[k] + ((uint32_t)rnd_shr((int32_t)((uint32_t)s.f[k] + (uint32_t)WM[k]), sh) << 16);
-            group_qrp<M, OMAX>(s, F, Wq, Wpc, dlpc, dlq, c0, nblk, sh, q, b,
+            group_qrp<M, OMAX>(s, F, Wq, Wpc, dlpc, c0, nblk, sh, q, b,
                                std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
             // the fine fields follow the group's net change: a full group flips group bit
             // GROUP_BITS - 1 once (every lower bit an even number of times); after a partial
@@ -1632,7 +1634,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
                     s.f[k] = (int32_t)(up ? (uint32_t)s.f[k] + (uint32_t)W3[k] : (uint32_t)s.f[k] - (uint32_t)W3[k]);
             }
         } else {
-            group_qrp_int32<M, OMAX>(s, Wq, dlpc, dlq, c0, nblk, sh, q, b);
+            group_qrp_int32<M, OMAX>(s, Wq, dlpc, c0, nblk, sh, q, b);
             if (groups_int32) ++*groups_int32;
         }
         if (b.count > QCAP - (1 << GROUP_BITS)) {   // room for one more group of records
