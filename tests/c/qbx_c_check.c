@@ -86,6 +86,31 @@ int main(void) {
     if (have != 2) return 10;
     printf("server %lld %llu %lld %llu\n", (long long)got[0].E_int, (unsigned long long)got[0].state,
            (long long)got[1].E_int, (unsigned long long)got[1].state);
+    /* seeded TSP instances built on the device from integer seeds (numpy's PCG64(seed)) */
+    uint64_t seeds[3] = {42, 7, 12345678901ULL};
+    int32_t cities[3] = {5, 4, 5};
+    qbx_pcg64_t st[3];
+    if (qbx_pcg64_from_seeds(3, seeds, st) != QBX_OK) return 11;
+    qbx_result_t rt[3];
+    if (qbx_batch_random_tsp(ctx, 3, st, cities, 0.0, 100.0, QBX_GEN_FLOAT32, &o, rt) != QBX_OK) {
+        fprintf(stderr, "tsp: %s\n", qbx_last_error());
+        return 12;
+    }
+    printf("tsp %.17g %llu\n", rt[0].E, (unsigned long long)rt[0].state);
+
+    /* a float problem with either float filter forced: the same answer; both at once is refused */
+    double F[N2 * N2];
+    for (int i = 0; i < N2 * N2; ++i) F[i] = (double)Q2[i] / 4.0;
+    qbx_opts_t op = {-1, 12, -1, QBX_FLAG_FLOAT_PACKED_FILTER}, oi = {-1, 12, -1, QBX_FLAG_FLOAT_INT32_FILTER};
+    qbx_result_t fp, fi;
+    if (qbx_solve_f64(ctx, F, N2, &op, &fp) != QBX_OK || qbx_solve_f64(ctx, F, N2, &oi, &fi) != QBX_OK) {
+        fprintf(stderr, "float: %s\n", qbx_last_error());
+        return 13;
+    }
+    if (fp.E != fi.E || fp.state != fi.state || fp.path != QBX_PATH_FLOAT_PAIR16 || fi.path != QBX_PATH_FLOAT) return 14;
+    qbx_opts_t both = {-1, 12, -1, QBX_FLAG_FLOAT_PACKED_FILTER | QBX_FLAG_FLOAT_INT32_FILTER};
+    if (qbx_solve_f64(ctx, F, N2, &both, &fp) != QBX_EINVAL) return 15;
+    printf("float %.17g %llu\n", fi.E, (unsigned long long)fi.state);
     qbx_close(ctx);
     return 0;
 }

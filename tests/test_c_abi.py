@@ -42,3 +42,9 @@ def test_c_program_matches_python_binding(gpu_ctx):
     assert lines["server"].split() == lines["batch"].split()
     c = gpu_ctx.solve_i32(_lcg_matrix(20, 777))
     assert lines["parts"].split() == [str(c.E_int), str(c.state)]
+    t = gpu_ctx.random_tsp_batch([42], [5], 0, 100, np.float32)[0]
+    E, s = lines["tsp"].split()
+    assert (float(E), int(s)) == (t.E, t.state)
+    f = gpu_ctx.solve_f64(_lcg_matrix(20, 777).astype(np.float64) / 4)
+    E, s = lines["float"].split()
+    assert (float(E), int(s)) == (f.E, f.state)
