@@ -1,27 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark: exhaustive QUBO solve throughput on one MI355X.
+"""Benchmark: exhaustive QUBO solving on one MI355X.
 
-Default workload (the headline metric): RandomQUBO-style n = 40, Q = triu(U) + triu(U,1)^T
-with U ~ integers[-127, 127] int16 (PCG64 seed 0), all 2^40 states enumerated exactly with
-int32 accumulation.  One step = one complete solve (prep + walk kernels) of that instance;
-Q is staged once per solve (3.2 KB) and the answer (16 B) read back.
+Headline (the default run): RandomQUBO-style n = 40, Q = triu(U) + triu(U,1)^T with
+U ~ integers[-127, 127] int16 (PCG64 seed 0), all 2^40 states enumerated exactly.  One
+step = one complete solve (prep + walk kernels) of that instance; Q is staged once per
+solve (3.2 KB) and the answer (16 B) read back.  The same JSON line also carries the
+metric's second clause and the other configurations, each timed in this run:
 
-    python bench.py [--steps K] [--warmup W] [--workload exhaustive|float|batch|serve|copy]
-                    [--no-cpu-baseline]
+  batch     batch_mix(4096, seed 0) -- MaxCut G(n, 1/2), n ~ U{16..32}, and integer TSP of
+            4-5 cities -- in one persistent launch (a captured HIP graph, replayed per
+            step) against one launch + sync per problem: solves/s, p50/p99 per-problem
+            latency of both forms, speedup, identical answers
+  serve     serving_requests(20000, seed 0) at 20 000 requests/s through the
+            continuous-batching server, against sequential plain solves of the same
+            arrivals: solves/s, p50/p99 latency, identical answers
+  cpu_c1    the reference's CPU configuration: QBF (numpy restatement) on n = 24 int32
+            RandomQUBO-style instances, seeds 0-2, end to end, exact against the goldens
+  float34   n = 34 float32 N(0,1) instance, FP64-exact argmin: states/s, |E - E_fp64|,
+            argmin agreement with the FP64 brute-force golden
 
-Other workloads: `float` (n = 34 N(0,1) float32 QUBO, FP64-exact argmin; `--float-n` for
-another size), `batch` (4096
-problems of uneven n in one launch vs one launch per problem), `serve` (Poisson request
-stream through the continuous-batching server vs sequential plain solves), `copy` (a device
-copy of known size, to calibrate the FETCH_SIZE / WRITE_SIZE counters).
+    python bench.py [--steps K] [--warmup W] [--workload exhaustive|float|maxcut|batch|serve|copy]
+                    [--headline-only] [--no-cpu-baseline]
 
-Prints ONE JSON line.  `value` is states/s for the whole job (one GPU) with Q resident in
-device memory (HIP-event time of each solve's kernels); `host_inclusive` gives the wall-clock
-rate with the per-solve host copies.  `roofline` prices
-the walk kernel against the gfx950 int32 VALU peak (CUs x 4 SIMD x 16 lanes x clock) with
-2 algorithmic ops per state (one energy update, one comparison); `cpu_baseline` is the
-numpy restatement of the reference QBF algorithm (oracle/qbf_numpy.py) timed on a bounded
-sample of the same instance on this host.
+`value` is states/s for the whole job (one GPU) with Q resident in device memory (HIP-event
+time of each solve's kernels); `host_inclusive` gives the wall-clock rate with the
+per-solve host copies.  `roofline` prices the walk against the gfx950 VALU issue rate
+(CUs x 4 SIMD x 16 lanes x clock lane-instructions per second): `frac` = states/s x the
+fewest VALU instructions per state the walk that ran can issue, / that peak.  The PMC
+instruction count and VALU busy of the same walk come from the committed profiles.
+`cpu_baseline` is the numpy restatement of the reference QBF algorithm
+(oracle/qbf_numpy.py) timed on a bounded sample of the same instance on this host.
 """
 from __future__ import annotations
 
@@ -38,13 +46,66 @@ sys.path.insert(0, str(ROOT / "evo-quantum-qubolib_amd"))
 
 import numpy as np  # noqa: E402
 
-OPS_PER_STATE = 2
+# The reference's "O(1) per state" (qbf.py:22): one energy update and one comparison.
+ALG_OPS_PER_STATE = 2
+PACKED_WALKS = ("pair16", "pair16_rebased", "float_pair16")
+METRIC = "states/s at n=40 int16 (exact); batched solves/s + p99 vs per-instance launch"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def isa_min_instr_per_state(walk: str, M: int) -> float:
+    """Fewest VALU instructions per state the walk can issue.  The inner block is a binary
+    trie over M bits: 2^M - 1 states are their parent plus one field plus one constant
+    (one v_add3_u32 each) and the 2^M states fold into the block minimum two at a time
+    (one v_min3_i32 per pair).  The packed walks hold two states per 32-bit word, so one
+    v_add3_u32 advances two states and one v_pk_minimum3_f16 folds four: half the count."""
+    per = ((1 << M) - 1 + (1 << (M - 1))) / float(1 << M)
+    return per / 2.0 if walk in PACKED_WALKS else per
+
+
+def _committed(name: str, key: str):
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    return json.loads(path.read_text()).get(key)
+
+
+def measured_traffic(key: str):
+    """HBM bytes per walk dispatch from the committed PMC summary (profiles/traffic.json):
+    FETCH_SIZE x fetch factor + WRITE_SIZE x write factor (KiB), or None."""
+    t = _committed("traffic.json", key)
+    if not t:
+        return None
+    return (t["fetch_kib"] * t["factor_fetch"] + t["write_kib"] * t["factor_write"]) * 1024.0
+
+
+def roofline(states_per_s: float, walk: str, M: int, devinfo: dict, key: str | None) -> dict:
+    """VALU roofline of the walk: achieved = states/s x the walk's ISA-minimal VALU
+    instructions per state; peak = the device's VALU issue rate in lane-instructions/s."""
+    peak = devinfo["valu_peak_ops"]
+    imin = isa_min_instr_per_state(walk, M)
+    achieved = states_per_s * imin
+    out = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "T VALU lane-instr/s",
+           "frac": achieved / peak,
+           "traffic": measured_traffic(key) if key else None,
+           "walk": walk, "isa_min_instr_per_state": imin, "ceiling_states_per_s": peak / imin,
+           "peak_basis": (f"{devinfo['compute_units']} CUs x 4 SIMD x 16 lanes x {devinfo['clock_khz'] / 1e6:.3f} GHz "
+                          "(hipGetDeviceProperties)"),
+           "alg_ops_rate": {"value": states_per_s * ALG_OPS_PER_STATE / 1e12, "unit": "T ops/s",
+                            "note": "2 algorithmic ops per state (one energy update, one comparison; qbf.py:22); "
+                                    "a rate, not a bound: one packed instruction retires several of them"}}
+    pmc = _committed("pmc.json", key) if key else None
+    if pmc:
+        out["pmc"] = {"valu_instr_per_state": pmc["valu_instr_per_state"], "valu_busy": pmc["valu_busy"],
+                      "issue_frac": states_per_s * pmc["valu_instr_per_state"] / peak, "source": pmc["source"],
+                      "note": "issue_frac = states/s (this run) x PMC VALU instructions per state / peak"}
+    return out
+
+
+# ------------------------------------------------------------------------ CPU baselines
 def cpu_baseline(Q, seconds_target: float = 12.0) -> dict:
     """Numpy restatement of QBF on the first K prefix steps of the same instance."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
@@ -109,16 +170,88 @@ def cpu_baseline_parallel(Q, seconds_target: float = 6.0) -> dict:
                        f"cache builds excluded")}
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per walk dispatch from the committed PMC summary (profiles/traffic.json):
-    (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) KiB, or None."""
-    path = ROOT / "profiles" / "traffic.json"
-    if not path.exists():
+def cpu_c1() -> dict:
+    """The reference's CPU configuration, end to end: QBF (numpy restatement, B = 20,
+    A = 4, qbf.py:188-230, 443-446) on the n = 24 int32 instances of seeds 0-2, exact
+    (E, argmin) against the brute-force goldens."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle.qbf_numpy import QBFNumpy
+    from qubo_lib_amd.problems.generators import symmetric_int_qubo
+    gold = json.loads((ROOT / "tests" / "golden" / "oracle_goldens.json").read_text())
+    rows = []
+    for seed in (0, 1, 2):
+        Q = symmetric_int_qubo(24, seed, -100, 100, np.int32)
+        t0 = time.perf_counter()
+        q = QBFNumpy(Q)
+        q.build_cache()
+        E, state, done, _ = q.run()
+        wall = time.perf_counter() - t0
+        g = gold[f"sym_int32_n24_s{seed}"]
+        E = int(round(float(E)))
+        rows.append({"seed": seed, "E": E, "state": int(state), "wall_s": wall, "states_per_s": 2.0 ** 24 / wall,
+                     "matches_golden": bool(E == g["E"] and int(state) == g["state"])})
+    wall = sum(r["wall_s"] for r in rows)
+    return {"workload": "QBF on RandomQUBO-style n=24 int32, Q = triu(U)+triu(U,1)^T, U ~ integers[-100,100], "
+                        "seeds 0-2, B=20 A=4 (2^24 states each), end to end incl. cache build",
+            "impl": "numpy QBF restatement (oracle/qbf_numpy.py)", "cores": 1, "host_cpu_count": os.cpu_count(),
+            "wall_s": wall, "states_per_s": 3 * 2.0 ** 24 / wall, "instances": rows,
+            "all_match_golden": all(r["matches_golden"] for r in rows)}
+
+
+# ------------------------------------------------------------------ exhaustive solves
+def _golden(key):
+    gold = ROOT / "tests" / "golden" / "oracle_goldens.json"
+    if not gold.exists():
         return None
-    t = json.loads(path.read_text()).get(workload)
-    if not t:
-        return None
-    return (t["fetch_kib"] * t["factor_fetch"] + t["write_kib"] * t["factor_write"]) * 1024.0
+    return json.loads(gold.read_text()).get(key)
+
+
+def _parity(E, state, key) -> dict:
+    parity = {"E": E, "state": state}
+    g = _golden(key)
+    if g is not None:
+        parity["oracle_state"] = g["state"]
+        parity["oracle_E"] = g["E"]
+        parity["abs_err_vs_oracle"] = abs(float(g["E"]) - float(E))
+        parity["matches_oracle"] = bool(g["state"] == state and
+                                        (g["E"] == E if isinstance(E, int) else abs(g["E"] - E) < 1e-9 * max(1, abs(E))))
+    return parity
+
+
+def _time_solves(prep, steps, warmup, kw):
+    from qubo_lib_amd import engine
+    for _ in range(warmup):
+        engine.solve_matrix(prep, **kw)
+    kms, results = [], []
+    # every libqbx call ends with hipStreamSynchronize on its stream, so the clock below
+    # brackets completed device work (torch is not loaded: it carries its own HIP runtime)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = engine.solve_matrix(prep, **kw)
+        kms.append(r.info["kernel_ms"])
+        results.append((r.E, r.state))
+    wall = time.perf_counter() - t0
+    assert len(set(results)) == 1, "answers differ between steps"
+    return r, kms, wall
+
+
+def float_leg(args, devinfo, n: int = 34, steps: int = 20, warmup: int = 3) -> dict:
+    """The float configuration: n = 34 N(0,1) float32 instance, FP64-exact answer."""
+    from qubo_lib_amd import engine
+    from qubo_lib_amd.problems.generators import symmetric_normal_qubo
+    Q = symmetric_normal_qubo(n, args.seed)
+    r, kms, wall = _time_solves(engine.prepare(Q), steps, warmup, {})
+    value = 2.0 ** n * steps / (float(np.sum(kms)) * 1e-3)
+    parity = _parity(r.E, r.state, f"sym_normal_n{n}_s{args.seed}")
+    return {"workload": f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: filter walk over quantized "
+                        "tables, FP64 exact re-walks", "steps": steps, "warmup": warmup,
+            "states_per_s": value, "kernel_ms_mean": float(np.mean(kms)), "ms_per_step": wall * 1e3 / steps,
+            "walk": r.info["walk"], "walk_bits": r.info["walk_bits"], "tiles": r.info["tiles"],
+            "abs_err_vs_fp64": parity.get("abs_err_vs_oracle"),
+            "argmin_agrees": parity.get("oracle_state") == r.state if "oracle_state" in parity else None,
+            "parity": parity,
+            "roofline": roofline(value, r.info["walk"], r.info["inner_bits"], devinfo,
+                                 "float" if n == 34 and args.seed == 0 else None)}
 
 
 def exhaustive(args, ctx, devinfo):
@@ -128,31 +261,24 @@ def exhaustive(args, ctx, devinfo):
         n = args.float_n
         Q = symmetric_normal_qubo(n, args.seed)
         workload = f"symmetric N(0,1) float32 QUBO n={n}, seed {args.seed}: filter walk over quantized tables, FP64 exact re-walk"
-        dtype = "f32"
+        dtype, key, gkey = "f32", ("float" if n == 34 else None), f"sym_normal_n{n}_s{args.seed}"
+        metric = f"states/s at n={n} float32 (FP64-exact argmin)"
     elif args.workload == "maxcut":
         n = 40
         Q = maxcut_gnp(n, 0.5, seed=args.seed)
         workload = (f"MaxCut G(n=40, p=0.5) unit weights (seed {args.seed}), 2^40 states, exact: energy bound "
                     f"<= 15359, packed pair walk (two 16-bit states per 32-bit op)")
-        dtype = "int32"
+        dtype, key, gkey = "int32", "maxcut", f"maxcut_n40_s{args.seed}"
+        metric = "states/s at n=40 MaxCut G(40, 0.5) (exact, packed pair walk)"
     else:
         n = 40
         Q = symmetric_int_qubo(n, args.seed, -127, 127, np.int16)
         workload = f"RandomQUBO n={n}, symmetric int16 U[-127,127] (seed {args.seed}), 2^{n} states, exact"
-        dtype = "int32"
+        dtype, key, gkey = "int32", "exhaustive", f"sym_int16_n40_s{args.seed}"
+        metric = METRIC
     prep = engine.prepare(Q)
     kw = dict(walk_bits=args.walk_bits, inner_bits=args.inner_bits)
-    for _ in range(args.warmup):
-        engine.solve_matrix(prep, **kw)
-    kms, results = [], []
-    # every libqbx call ends with hipStreamSynchronize on its stream, so the clock below
-    # brackets completed device work (torch is not loaded: it carries its own HIP runtime)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = engine.solve_matrix(prep, **kw)
-        kms.append(r.info["kernel_ms"])
-        results.append((r.E, r.state))
-    wall = time.perf_counter() - t0
+    r, kms, wall = _time_solves(prep, args.steps, args.warmup, kw)
     states = 2.0 ** n
     ms_per_step = wall * 1000.0 / args.steps
     kernel_ms = float(np.mean(kms))
@@ -161,16 +287,9 @@ def exhaustive(args, ctx, devinfo):
     # 3.2 KB host-to-device copy and 32 B result read-back is reported beside it
     value = states * args.steps / (float(np.sum(kms)) * 1e-3)
     value_host = states * args.steps / wall
-    peak = devinfo["valu_peak_ops"]
-    # the packed walks do two 16-bit operations per lane and instruction: their ceiling is
-    # the packed 16-bit VALU rate, twice the int32 one
-    packed = r.info["walk"] in ("pair16", "pair16_rebased", "float_pair16")
-    if packed:
-        peak *= 2
-    achieved = states * OPS_PER_STATE / (kernel_ms * 1e-3)
-    assert len(set(results)) == 1, "answers differ between steps"
-    E, state = results[0]
-    parity = {"E": E, "state": state}
+    packed = r.info["walk"] in PACKED_WALKS
+    E, state = r.E, r.state
+    parity = _parity(E, state, gkey)
     if args.workload == "maxcut":
         # parity against the int32 walk on the same problem scaled out of the packed walks'
         # range (same argmin, energies times k)
@@ -179,21 +298,8 @@ def exhaustive(args, ctx, devinfo):
         parity.update(walk=r.info["walk"], int32_walk={"E": ref.E, "state": ref.state, "scale": k,
                                                        "walk": ref.info["walk"], "kernel_ms": ref.info["kernel_ms"]},
                       matches_int32_walk=bool(ref.E == E * k and ref.state == state))
-    gold = ROOT / "tests" / "golden" / "oracle_goldens.json"
-    key = {"float": f"sym_normal_n{n}_s{args.seed}", "maxcut": f"maxcut_n40_s{args.seed}"}.get(
-        args.workload, f"sym_int16_n40_s{args.seed}")
-    if gold.exists() and key:
-        g = json.loads(gold.read_text()).get(key)
-        if g is not None:
-            parity["oracle_state"] = g["state"]
-            parity["oracle_E"] = g["E"]
-            parity["abs_err_vs_oracle"] = abs(float(g["E"]) - float(E))
-            parity["matches_oracle"] = bool(g["state"] == state and
-                                            (g["E"] == E if isinstance(E, int) else abs(g["E"] - E) < 1e-9 * max(1, abs(E))))
     line = {
-        "metric": {"float": f"states/s at n={n} float32 (FP64-exact argmin)",
-                   "maxcut": "states/s at n=40 MaxCut G(40, 0.5) (exact, packed pair walk)"}.get(
-                       args.workload, "states/s at n=40 int16 (exact)"),
+        "metric": metric,
         "value": value,
         "unit": "states/s",
         "n_gpus": 1,
@@ -212,20 +318,13 @@ def exhaustive(args, ctx, devinfo):
                    "kernel_ms_mean": kernel_ms, "tuning": str(ctx.tuning_file.name) if ctx.tuning_file else "built-in rule",
                    "device": devinfo["name"], "compute_units": devinfo["compute_units"],
                    "clock_mhz": devinfo["clock_khz"] / 1000.0},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TOP/s",
-                     "frac": achieved / peak,
-                     # PMC traffic is committed for the default instances only
-                     "traffic": (None if args.workload == "float" and n != 34 else
-                                 measured_traffic({"float": "float", "maxcut": "maxcut"}.get(args.workload, "exhaustive"))),
-                     "ops_per_state": OPS_PER_STATE,
-                     "note": "kernel time = HIP events around prep+walk on the library stream" +
-                             (f"; walk {r.info['walk']}: peak = packed 16-bit VALU ops (2 per lane per instruction)"
-                              if packed else "")},
+        "roofline": roofline(value, r.info["walk"], r.info["inner_bits"], devinfo, key),
         "parity": parity,
         "host_inclusive": {"value": value_host, "unit": "states/s",
                            "note": "wall clock around the K solve calls: staging Q from host memory, "
                                    "launches, result read-back and synchronisation included"},
     }
+    line["roofline"]["note"] = "kernel time = HIP events around prep+walk on the library stream"
     if args.shard_parts > 1:
         line["shard"] = shard_shares(prep, args.shard_parts, r, kw)
     if args.cpu_baseline:
@@ -254,8 +353,9 @@ def shard_shares(prep, parts: int, full, kw) -> dict:
             "merged_matches_full": (merged.E, merged.state) == (full.E, full.state)}
 
 
-def batch(args, ctx, devinfo):
-    """4096-problem uneven mix: one persistent launch vs one launch per problem."""
+# --------------------------------------------------------------------------- batches
+def batch_leg(args, ctx, steps: int, warmup: int, plain: bool = True, generated: bool = True) -> dict:
+    """4096-problem uneven mix: one persistent launch (graph replay) vs one launch per problem."""
     from qubo_lib_amd import engine
     from qubo_lib_amd.problems.generators import batch_mix
     probs, kinds = batch_mix(args.batch_count, seed=args.seed)
@@ -263,61 +363,70 @@ def batch(args, ctx, devinfo):
     preps = [engine.prepare(q) for q in Qs]
     mats = [p.matrix for p in preps]
     packed = ctx.pack(mats, np.int32)
-    # batched form: the batch shape captured once in a HIP graph, replayed per step
+    # batched form: the batch shape captured once in a HIP graph, replayed per step; each
+    # problem's latency in this form is the wall time of the replay that solves it
     plan = ctx.batch_plan([m.shape[0] for m in mats], kind="i32")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plan.run(packed=packed)
+    walls = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
+        t1 = time.perf_counter()
         res = plan.run(packed=packed)
-    wall_b = (time.perf_counter() - t0) / args.steps
-    graph_ms = float(np.mean([res[0].kernel_ms]))
+        walls.append(time.perf_counter() - t1)
+    wall_b = (time.perf_counter() - t0) / steps
+    graph_ms = float(res[0].kernel_ms)
     plan.close()
     # the same launches issued eagerly (no graph)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ctx.batch(mats, kind="i32", packed=packed)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         res_e = ctx.batch(mats, kind="i32", packed=packed)
-    wall_e = (time.perf_counter() - t0) / args.steps
+    wall_e = (time.perf_counter() - t0) / steps
     # plain form: one C-ABI solve (launch + sync) per problem, same kernels and GPU
-    lat = []
-    plain = []
-    mats_plain = mats if args.batch_plain else []
+    lat, out_p = [], []
     t0 = time.perf_counter()
-    for m in mats_plain:
+    for m in (mats if plain else []):
         t1 = time.perf_counter()
-        plain.append(ctx.solve_i32(m, walk_bits=None))
+        out_p.append(ctx.solve_i32(m, walk_bits=None))
         lat.append(time.perf_counter() - t1)
     wall_p = time.perf_counter() - t0
-    if not args.batch_plain:          # profiling runs: batched launches only
-        plain, lat, wall_p = res, [float("nan")], float("nan")
+    if not plain:          # profiling runs: batched launches only
+        out_p, lat, wall_p = res, [float("nan")], float("nan")
     same = all((a.E_int, a.state) == (b.E_int, b.state) == (e.E_int, e.state)
-               for a, b, e in zip(res, plain, res_e))
+               for a, b, e in zip(res, out_p, res_e))
     states = float(sum(2.0 ** q.shape[0] for q in Qs))
-    drawn = device_drawn_random_batch(args, ctx)
-    drawn_tsp = device_built_tsp_batch(args, ctx)
-    return {
-        "metric": "batched solves/s (4096-instance uneven mix)",
-        "value": len(Qs) / wall_b,
-        "unit": "solves/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall_b * 1000.0,
-        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"batch_mix({len(Qs)}, seed={args.seed}): MaxCut G(n,0.5) n~U{{16..32}} / int TSP 4-5 cities",
-                   "states": states, "form": "one HIP graph replay per step (copy in, prep, persistent walks, copy out)",
-                   "tuning": str(ctx.tuning_file.name) if ctx.tuning_file else "built-in rule"},
-        "graph_ms": graph_ms,
-        "eager": {"solves_per_s": len(Qs) / wall_e, "ms_per_step": wall_e * 1000.0},
-        "plain": {"solves_per_s": len(Qs) / wall_p, "wall_s": wall_p, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-                  "p99_ms": float(np.percentile(lat, 99) * 1e3)},
+    out = {
+        "workload": f"batch_mix({len(Qs)}, seed={args.seed}): MaxCut G(n,0.5) n~U{{16..32}} / int TSP 4-5 cities",
+        "count": len(Qs), "states": states, "steps": steps, "warmup": warmup,
+        "form": "one HIP graph replay per step (copy in, prep, persistent walks, copy out)",
+        "solves_per_s": len(Qs) / wall_b, "ms_per_step": wall_b * 1e3, "graph_ms": graph_ms,
+        "p50_ms": float(np.percentile(walls, 50) * 1e3), "p99_ms": float(np.percentile(walls, 99) * 1e3),
+        "eager": {"solves_per_s": len(Qs) / wall_e, "ms_per_step": wall_e * 1e3},
+        "plain": {"solves_per_s": len(Qs) / wall_p, "wall_s": wall_p,
+                  "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                  "form": "one qbx_solve_i32 (launch + sync) per problem, same kernels, tuned single tiling"},
         "speedup_vs_plain": wall_p / wall_b,
         "identical_answers": same,
-        "random_qubo_drawn_on_device": drawn,
-        "tsp_built_on_device": drawn_tsp,
         "states_per_s": states / wall_b,
+        "tuning": str(ctx.tuning_file.name) if ctx.tuning_file else "built-in rule",
     }
+    if generated:
+        out["random_qubo_drawn_on_device"] = device_drawn_random_batch(args, ctx)
+        out["tsp_built_on_device"] = device_built_tsp_batch(args, ctx)
+    return out
+
+
+def batch(args, ctx, devinfo):
+    b = batch_leg(args, ctx, args.steps, args.warmup, plain=args.batch_plain)
+    line = {"metric": "batched solves/s (4096-instance uneven mix)", "value": b["solves_per_s"], "unit": "solves/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms_per_step"],
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic", "config": {"workload": b["workload"], "states": b["states"], "form": b["form"],
+                                            "tuning": b["tuning"]}}
+    line.update({k: v for k, v in b.items() if k not in line})
+    return line
 
 
 def device_drawn_random_batch(args, ctx, count: int = 4096) -> dict:
@@ -370,58 +479,26 @@ def device_built_tsp_batch(args, ctx, count: int = 4096) -> dict:
             "identical_answers": same}
 
 
-def serve(args, ctx, devinfo):
-    """Poisson request stream (n in 20..36, P(n) ~ 2^-(n-20); half symmetric int16
-    RandomQUBO in [-127,127], half MaxCut G(n,0.5)) served by the continuous-batching
-    server at several arrival rates, and by sequential plain solves at the headline rate."""
-    from qubo_lib_amd import engine
-    from qubo_lib_amd.problems.generators import serving_requests
+# --------------------------------------------------------------------------- serving
+def _serve_at(preps, arrivals, rate_scale, answers, warm=64):
+    """Submit every request at its arrival time (scaled) to a fresh server; returns the
+    rate row.  answers[j] collects request j's (E, state)."""
     from qubo_lib_amd.serve import Server
-    count = args.serve_count
-    base_rate = 20000.0
-    reqs = serving_requests(count, base_rate, seed=args.seed)
-    preps = [engine.prepare(q) for _, q, _ in reqs]
-    arrivals = np.array([t for t, _, _ in reqs])
-    rows = []
-    answers = {}
-    for rate in args.serve_rates:
-        arr = arrivals * (base_rate / rate)
-        with Server(max_n=40) as srv:
-            # warm-up: kernels loaded, slots touched
-            for p in preps[:64]:
-                srv.submit(p)
-            w = 0
-            while w < 64:
-                w += len(srv.poll(wait_ms=20))
-            t0 = time.perf_counter()
-            ids = []
-            for a, p in zip(arr, preps):
-                while time.perf_counter() - t0 < a:
-                    pass
-                ids.append(srv.submit(p))
-            got = {}
-            while len(got) < len(ids):
-                for r in srv.poll(wait_ms=20):
-                    got[r["id"]] = r
-            wall = time.perf_counter() - t0
-            st = srv.stats()
-        lat = np.array([got[i]["t_done_ms"] - got[i]["t_submit_ms"] for i in ids])
-        for j, i in enumerate(ids):
-            answers.setdefault(j, set()).add((got[i]["E"], got[i]["state"]))
-        rows.append({"rate_per_s": rate, "solves_per_s": len(ids) / wall, "p50_ms": float(np.percentile(lat, 50)),
-                     "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "wall_s": wall,
-                     "steps": int(st["steps"]), "device_busy_ms": st["busy_ms"]})
-        log(f"serve rate={rate:.0f}/s -> {rows[-1]}")
-    # capacity: every request submitted at once, in one submit_many call
+    arr = arrivals * rate_scale
     with Server(max_n=40) as srv:
-        for p in preps[:64]:
+        # warm-up: kernels loaded, slots touched
+        for p in preps[:warm]:
             srv.submit(p)
         w = 0
-        while w < 64:
+        while w < warm:
             w += len(srv.poll(wait_ms=20))
+        st0 = srv.stats()
         t0 = time.perf_counter()
-        ids = srv.submit_many(preps)
-        t_sub = time.perf_counter() - t0
+        ids = []
+        for a, p in zip(arr, preps):
+            while time.perf_counter() - t0 < a:
+                pass
+            ids.append(srv.submit(p))
         got = {}
         while len(got) < len(ids):
             for r in srv.poll(wait_ms=20):
@@ -431,38 +508,91 @@ def serve(args, ctx, devinfo):
     lat = np.array([got[i]["t_done_ms"] - got[i]["t_submit_ms"] for i in ids])
     for j, i in enumerate(ids):
         answers.setdefault(j, set()).add((got[i]["E"], got[i]["state"]))
-    burst = {"solves_per_s": len(ids) / wall, "submit_us_per_request": t_sub / len(ids) * 1e6,
-             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-             "steps": int(st["steps"]), "device_busy_ms": st["busy_ms"]}
-    log(f"serve burst -> {burst}")
-    # sequential plain solves (one blocking launch each) on the same arrivals at the base rate
-    t0 = time.perf_counter()
-    lat = []
-    for j, (a, p) in enumerate(zip(arrivals, preps)):
-        while time.perf_counter() - t0 < a:
-            pass
-        r = engine.solve_matrix(p)
-        lat.append((time.perf_counter() - t0 - a) * 1e3)
-        answers.setdefault(j, set()).add((r.E, r.state))
-    wall = time.perf_counter() - t0
-    lat = np.array(lat)
-    seq = {"rate_per_s": base_rate, "solves_per_s": len(lat) / wall, "p50_ms": float(np.percentile(lat, 50)),
-           "p99_ms": float(np.percentile(lat, 99)), "wall_s": wall}
-    same = all(len(v) == 1 for v in answers.values())
-    head = max(rows, key=lambda r: r["solves_per_s"] if r["p99_ms"] < 5.0 else 0.0)
-    return {
-        "metric": "serving solves/s (Poisson arrivals, n 20..36) with p50/p99 latency",
-        "value": head["solves_per_s"], "unit": "solves/s",
-        "n_gpus": 1, "steps": count, "warmup": 64, "ms_per_step": 1000.0 / head["solves_per_s"],
-        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"serving_requests({count}, seed={args.seed}): n in 20..36, P(n)~2^-(n-20), "
-                               "half int16 RandomQUBO [-127,127], half MaxCut G(n,0.5)",
-                   "headline_rate_per_s": head["rate_per_s"]},
-        "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
-        "rates": rows, "burst": burst, "sequential": seq,
-        "identical_answers": same,
-    }
+    return {"solves_per_s": len(ids) / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "wall_s": wall, "steps": int(st["steps"] - st0["steps"]),
+            "device_busy_ms": st["busy_ms"] - st0["busy_ms"]}
+
+
+def serve_leg(args, rates, count: int, burst: bool = True, sequential: bool = True) -> dict:
+    """Poisson request stream (n in 20..36, P(n) ~ 2^-(n-20); half symmetric int16
+    RandomQUBO in [-127,127], half MaxCut G(n,0.5)) served by the continuous-batching
+    server at the given arrival rates, and by sequential plain solves at the first rate."""
+    from qubo_lib_amd import engine
+    from qubo_lib_amd.problems.generators import serving_requests
+    from qubo_lib_amd.serve import Server
+    base_rate = 20000.0
+    reqs = serving_requests(count, base_rate, seed=args.seed)
+    preps = [engine.prepare(q) for _, q, _ in reqs]
+    arrivals = np.array([t for t, _, _ in reqs])
+    rows = []
+    answers = {}
+    for rate in rates:
+        row = {"rate_per_s": rate}
+        row.update(_serve_at(preps, arrivals, base_rate / rate, answers))
+        rows.append(row)
+        log(f"serve rate={rate:.0f}/s -> {row}")
+    out = {"workload": f"serving_requests({count}, seed={args.seed}): Poisson arrivals, n in 20..36, "
+                       "P(n)~2^-(n-20), half int16 RandomQUBO [-127,127], half MaxCut G(n,0.5)",
+           "rates": rows}
+    if burst:
+        # capacity: every request submitted at once, in one submit_many call
+        with Server(max_n=40) as srv:
+            for p in preps[:64]:
+                srv.submit(p)
+            w = 0
+            while w < 64:
+                w += len(srv.poll(wait_ms=20))
+            st0 = srv.stats()
+            t0 = time.perf_counter()
+            ids = srv.submit_many(preps)
+            t_sub = time.perf_counter() - t0
+            got = {}
+            while len(got) < len(ids):
+                for r in srv.poll(wait_ms=20):
+                    got[r["id"]] = r
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        lat = np.array([got[i]["t_done_ms"] - got[i]["t_submit_ms"] for i in ids])
+        for j, i in enumerate(ids):
+            answers.setdefault(j, set()).add((got[i]["E"], got[i]["state"]))
+        out["burst"] = {"solves_per_s": len(ids) / wall, "submit_us_per_request": t_sub / len(ids) * 1e6,
+                        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                        "wall_ms": wall * 1e3, "steps": int(st["steps"] - st0["steps"]),
+                        "device_busy_ms": st["busy_ms"] - st0["busy_ms"]}
+        log(f"serve burst -> {out['burst']}")
+    if sequential:
+        # sequential plain solves (one blocking launch each) on the same arrivals
+        arr = arrivals * (base_rate / rates[0])
+        t0 = time.perf_counter()
+        lat = []
+        for j, (a, p) in enumerate(zip(arr, preps)):
+            while time.perf_counter() - t0 < a:
+                pass
+            r = engine.solve_matrix(p)
+            lat.append((time.perf_counter() - t0 - a) * 1e3)
+            answers.setdefault(j, set()).add((r.E, r.state))
+        wall = time.perf_counter() - t0
+        lat = np.array(lat)
+        out["sequential"] = {"rate_per_s": rates[0], "solves_per_s": len(lat) / wall,
+                             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                             "wall_s": wall}
+    out["identical_answers"] = all(len(v) == 1 for v in answers.values())
+    return out
+
+
+def serve(args, ctx, devinfo):
+    s = serve_leg(args, args.serve_rates, args.serve_count)
+    head = max(s["rates"], key=lambda r: r["solves_per_s"] if r["p99_ms"] < 5.0 else 0.0)
+    line = {"metric": "serving solves/s (Poisson arrivals, n 20..36) with p50/p99 latency",
+            "value": head["solves_per_s"], "unit": "solves/s",
+            "n_gpus": 1, "steps": args.serve_count, "warmup": 64, "ms_per_step": 1000.0 / head["solves_per_s"],
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": "int32", "data": "synthetic",
+            "config": {"workload": s["workload"], "headline_rate_per_s": head["rate_per_s"]},
+            "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"]}
+    line.update({k: v for k, v in s.items() if k not in line})
+    return line
 
 
 def copy_calibration(args):
@@ -496,6 +626,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["exhaustive", "float", "maxcut", "batch", "serve", "copy"],
                     default="exhaustive")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="exhaustive workload: skip the batch / serve / cpu_c1 / float34 legs")
     ap.add_argument("--serve-count", type=int, default=20000)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[2000.0, 20000.0, 100000.0])
     ap.add_argument("--copy-mib", type=int, default=256)
@@ -528,6 +660,16 @@ def main():
         line = serve(args, ctx, devinfo)
     else:
         line = exhaustive(args, ctx, devinfo)
+        if args.workload == "exhaustive" and not args.headline_only:
+            # the metric's second clause and the other configurations, timed in this run
+            log("batch: batch_mix(4096) one launch vs one launch per problem ...")
+            line["batch"] = batch_leg(args, ctx, steps=max(args.steps, 5), warmup=max(args.warmup, 2))
+            log("serve: 20000 Poisson requests at 20000/s ...")
+            line["serve"] = serve_leg(args, [20000.0], 20000)
+            log("float34: n=34 float32 ...")
+            line["float34"] = float_leg(args, devinfo)
+            log("cpu_c1: numpy QBF at n=24 int32, seeds 0-2 ...")
+            line["cpu_c1"] = cpu_c1()
     print(json.dumps(line), flush=True)
 
 

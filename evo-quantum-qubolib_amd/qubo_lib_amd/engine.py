@@ -88,9 +88,14 @@ def prepare(Q) -> Prepared:
             raise ValueError("Q contains non-finite values")
         if np.all(Qf == np.rint(Qf)) and np.abs(Qf).max(initial=0.0) < 2.0 ** 53:
             # integer-valued float Q (e.g. MaxCut's default float32 matrices): the energies
-            # are integers, so the exact int32 walk gives the same answer; E stays a float
-            p = prepare(Qf.astype(np.int64))
-            if p.kind == "i32":
+            # are integers, so the exact int32 walk gives the same answer; E stays a float.
+            # Integer bounds beyond the exact integer paths fall through to the float path,
+            # which solves any finite Q.
+            try:
+                p = prepare(Qf.astype(np.int64))
+            except OverflowError:
+                p = None
+            if p is not None and p.kind == "i32":
                 return Prepared(p.n, "i32", False, p.matrix, p.bound)
         bound = float(np.abs(np.diag(Qf)).sum() + np.abs(np.triu(Qf, 1) + np.triu(Qf.T, 1)).sum())
         return Prepared(n, "f64", False, np.ascontiguousarray(Qf), bound)

@@ -26,6 +26,11 @@ def test_path_selection():
     # integer-valued floats take the exact int32 walk (energies stay floats)
     g = engine.prepare(np.eye(3, dtype=np.float32))
     assert g.kind == "i32" and not g.integer
+    # integer-valued floats whose integer bound is 2^53 or more are still solvable: the
+    # float path takes them (the integer paths would refuse them)
+    h = engine.prepare(np.full((40, 40), 1e14))
+    assert h.kind == "f64" and not h.integer and h.bound >= 2.0 ** 53
+    assert engine.prepare(np.full((40, 40), 1e14) + 0.5).kind == "f64"
     with pytest.raises(ValueError):
         engine.prepare(np.array([[np.nan]]))
     with pytest.raises(ValueError):
