@@ -50,6 +50,9 @@ import numpy as np  # noqa: E402
 ALG_OPS_PER_STATE = 2
 PACKED_WALKS = ("pair16", "pair16_rebased", "float_pair16")
 METRIC = "states/s at n=40 int16 (exact); batched solves/s + p99 vs per-instance launch"
+# host threads for the device='cpu' backend: OMP_NUM_THREADS as given at start, else the
+# host's cores (at most 16, the host share of one GPU)
+HOST_THREADS = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
 
 
 def log(*a):
@@ -191,11 +194,33 @@ def cpu_c1() -> dict:
         rows.append({"seed": seed, "E": E, "state": int(state), "wall_s": wall, "states_per_s": 2.0 ** 24 / wall,
                      "matches_golden": bool(E == g["E"] and int(state) == g["state"])})
     wall = sum(r["wall_s"] for r in rows)
+    # the product's own device='cpu' backend (libqbx_cpu.so, OpenMP) on the same instances,
+    # through the drop-in solver, end to end
+    from qubo_lib_amd import _native
+    from qubo_lib_amd.problems import Just_Q
+    from qubo_lib_amd.solvers import QBF
+    _native.get_backend("cpu").threads = HOST_THREADS
+    prod = []
+    for seed in (0, 1, 2):
+        Q = symmetric_int_qubo(24, seed, -100, 100, np.int32)
+        QBF().solve(Just_Q(Q), device="cpu")                # warm-up (thread pool)
+        t0 = time.perf_counter()
+        sol = QBF().solve(Just_Q(Q), device="cpu")
+        wall_p = time.perf_counter() - t0
+        g = gold[f"sym_int32_n24_s{seed}"]
+        prod.append({"seed": seed, "wall_s": wall_p, "walk_ms": sol.kernel_ms,
+                     "matches_golden": bool(sol.best_E == g["E"] and
+                                            int("".join(map(str, sol.best_X)), 2) == g["state"])})
+    wall_p = sum(r["wall_s"] for r in prod)
     return {"workload": "QBF on RandomQUBO-style n=24 int32, Q = triu(U)+triu(U,1)^T, U ~ integers[-100,100], "
                         "seeds 0-2, B=20 A=4 (2^24 states each), end to end incl. cache build",
             "impl": "numpy QBF restatement (oracle/qbf_numpy.py)", "cores": 1, "host_cpu_count": os.cpu_count(),
             "wall_s": wall, "states_per_s": 3 * 2.0 ** 24 / wall, "instances": rows,
-            "all_match_golden": all(r["matches_golden"] for r in rows)}
+            "all_match_golden": all(r["matches_golden"] for r in rows),
+            "product_cpu": {"impl": "QBF().solve(p, device='cpu'): libqbx_cpu.so, the GPU's lane walk built for "
+                                    "the host, OpenMP over lanes", "threads": HOST_THREADS,
+                            "wall_s": wall_p, "states_per_s": 3 * 2.0 ** 24 / wall_p, "instances": prod,
+                            "all_match_golden": all(r["matches_golden"] for r in prod)}}
 
 
 # ------------------------------------------------------------------ exhaustive solves

@@ -1,6 +1,8 @@
 """Build step for the native parts of qubo_lib_amd (run by __graft_entry__.build()).
 
   libqbx.so      hipcc, gfx950: kernels + C ABI (include/qbx.h)  -> qubo_lib_amd/libqbx.so
+  libqbx_cpu.so  g++ -fopenmp: the same lane walk on the host CPU, the device='cpu' backend
+                 (include/qbx_cpu.h)                              -> qubo_lib_amd/libqbx_cpu.so
   libqbx_emu.so  g++: host build of the same lane walk, loaded only by the CPU test
                  suite to check the walk's indexing without a GPU -> tests/_build/
   qbx_c_check    gcc -std=c99: a plain C caller of the C ABI, run by a GPU test
@@ -20,6 +22,7 @@ REPO = HERE.parent
 CSRC = HERE / "csrc"
 INCLUDE = REPO / "include"
 LIB_OUT = HERE / "qubo_lib_amd" / "libqbx.so"
+CPU_OUT = HERE / "qubo_lib_amd" / "libqbx_cpu.so"
 EMU_OUT = REPO / "tests" / "_build" / "libqbx_emu.so"
 C_CHECK_OUT = REPO / "tests" / "_build" / "qbx_c_check"
 
@@ -50,6 +53,16 @@ def build_lib(out: Path = LIB_OUT) -> Path:
     return out
 
 
+def build_cpu(out: Path = CPU_OUT) -> Path:
+    """The device='cpu' backend: generic x86-64 code (it also runs on the GPU box's host)."""
+    out.parent.mkdir(parents=True, exist_ok=True)
+    tmp = out.with_suffix(".so.tmp")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wno-unknown-pragmas",
+          f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "qbx_cpu.cpp")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_emu(out: Path = EMU_OUT) -> Path:
     out.parent.mkdir(parents=True, exist_ok=True)
     # no floating-point contraction: the generator's float scaling is one rounded multiply
@@ -70,7 +83,7 @@ def build_c_check(out: Path = C_CHECK_OUT, lib: Path = LIB_OUT) -> Path:
 
 
 def build_all() -> list[Path]:
-    return [build_lib(), build_emu(), build_c_check()]
+    return [build_lib(), build_cpu(), build_emu(), build_c_check()]
 
 
 if __name__ == "__main__":

@@ -1,7 +1,9 @@
-"""ctypes binding of libqbx.so, the gfx950 exhaustive-QUBO library (C ABI in include/qbx.h).
+"""ctypes bindings of libqbx.so, the gfx950 exhaustive-QUBO library (C ABI in include/qbx.h),
+and of libqbx_cpu.so, the same lane walk built for the host (include/qbx_cpu.h), which
+serves device='cpu'.
 
-This module is the only place the product touches native code.  There is no CPU
-fallback: if the library is missing or no GPU is visible, every entry point raises.
+This module is the only place the product touches native code.  A 'cuda' device never
+falls back to the CPU: if libqbx.so is missing or no GPU is visible, its entry points raise.
 """
 from __future__ import annotations
 
@@ -15,10 +17,11 @@ import numpy as np
 __all__ = [
     "LIB_PATH", "NativeError", "Context", "Opts", "Result", "DevInfo", "ServerOpts", "RequestResult",
     "ServerStats", "Pcg64State", "pcg64_states", "GEN_DTYPES", "BatchPlan", "KINDS", "MODES", "load_library", "get_context", "read_tuning",
-    "tuning_path", "PATHS", "PAIR_BOUND",
+    "tuning_path", "PATHS", "PAIR_BOUND", "CpuBackend", "load_cpu_library", "get_backend", "parse_device",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libqbx.so"
+CPU_LIB_PATH = Path(__file__).resolve().parent / "libqbx_cpu.so"
 TUNING_DIR = Path(__file__).resolve().parent / "tuning"
 
 QBX_OK = 0
@@ -571,3 +574,119 @@ def get_context(device: int = 0) -> Context:
             c = Context(device)
             _contexts[device] = c
         return c
+
+
+# ------------------------------------------------------------------ device='cpu' backend
+_CPU_SIGNATURES = {
+    "qbx_cpu_solve_i32": (ctypes.c_int, [_P(ctypes.c_int32), ctypes.c_int, _P(Opts), ctypes.c_int, _P(Result)]),
+    "qbx_cpu_solve_f64": (ctypes.c_int, [_P(ctypes.c_double), ctypes.c_int, _P(Opts), ctypes.c_int, _P(Result)]),
+    "qbx_cpu_last_error": (ctypes.c_char_p, []),
+}
+CPU_EXPORTED_SYMBOLS = tuple(_CPU_SIGNATURES)
+_cpu_lib = None
+
+
+def load_cpu_library(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load libqbx_cpu.so (built in-tree by __graft_entry__.build()).  Raises if it is missing."""
+    global _cpu_lib
+    with _lib_lock:
+        if _cpu_lib is not None and path is None:
+            return _cpu_lib
+        p = Path(path) if path is not None else CPU_LIB_PATH
+        if not p.exists():
+            raise NativeError(QBX_EINVAL, f"{p} not found: build it with __graft_entry__.build()")
+        lib = ctypes.CDLL(str(p))
+        for name, (res, args) in _CPU_SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _cpu_lib = lib
+        return lib
+
+
+def _check_cpu(lib, rc: int):
+    if rc == QBX_OK:
+        return
+    msg = lib.qbx_cpu_last_error().decode(errors="replace")
+    if rc == QBX_EINVAL:
+        raise ValueError(msg)
+    if rc == QBX_ERANGE:
+        raise OverflowError(msg)
+    raise NativeError(rc, msg)
+
+
+class CpuBackend:
+    """device='cpu': libqbx_cpu.so runs the GPU kernels' lane walk on the host, OpenMP
+    threads over lanes (threads = 0: OpenMP's default).  Same answers as the GPU; the
+    same solve_i32 / solve_f64 / batch calls as Context."""
+
+    device = "cpu"
+
+    def __init__(self, threads: int = 0):
+        self.lib = load_cpu_library()
+        self.threads = int(threads)
+        self.lock = threading.Lock()
+        self.tuning_file = None
+
+    def _solve(self, fn, Q, dtype, ptr, kw) -> Result:
+        Q = np.ascontiguousarray(Q, dtype=dtype)
+        out = Result()
+        o = _opts(**kw)
+        with self.lock:
+            rc = fn(Q.ctypes.data_as(_P(ptr)), Q.shape[0], ctypes.byref(o), self.threads, ctypes.byref(out))
+        _check_cpu(self.lib, rc)
+        return out
+
+    def solve_i32(self, Q: np.ndarray, **kw) -> Result:
+        return self._solve(self.lib.qbx_cpu_solve_i32, Q, np.int32, ctypes.c_int32, kw)
+
+    def solve_f64(self, Q: np.ndarray, **kw) -> Result:
+        return self._solve(self.lib.qbx_cpu_solve_f64, Q, np.float64, ctypes.c_double, kw)
+
+    def batch(self, Qs, kind: str = "i32", packed=None, **kw):
+        """One solve after another (each solve uses every thread); a problem outside its
+        path's range gets status QBX_ERANGE, as in Context.batch."""
+        if packed is not None:
+            data, offsets, ns = packed
+            Qs = [data[o:o + n * n].reshape(n, n) for o, n in zip(offsets, ns)]
+        fn = self.solve_i32 if kind == "i32" else self.solve_f64
+        out = []
+        for Q in Qs:
+            try:
+                out.append(fn(Q, **kw))
+            except OverflowError:
+                r = Result()
+                r.n, r.status = np.asarray(Q).shape[0], QBX_ERANGE
+                out.append(r)
+        return out
+
+
+_cpu_backend = None
+
+
+def parse_device(device) -> int | str:
+    """'cpu' -> 'cpu'; 'cuda' -> 0; 'cuda:N' / N -> N (a HIP device).  ValueError otherwise."""
+    if isinstance(device, (int, np.integer)) and not isinstance(device, bool):
+        return int(device)
+    if isinstance(device, str):
+        if device == "cpu":
+            return "cpu"
+        if device[:4] == "cuda":
+            if len(device) == 4:
+                return 0
+            if device[4] == ":" and device[5:].isdigit():
+                return int(device[5:])
+    raise ValueError(f"Unknown device name: {device}")
+
+
+def get_backend(device=0):
+    """The solver backend of a device: CpuBackend for 'cpu', else the HIP device's Context."""
+    global _cpu_backend
+    dev = parse_device(device)
+    if dev == "cpu":
+        with _ctx_lock:
+            if _cpu_backend is None:
+                _cpu_backend = CpuBackend()
+            return _cpu_backend
+    return get_context(dev)

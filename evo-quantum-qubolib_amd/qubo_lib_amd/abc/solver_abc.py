@@ -2,7 +2,8 @@
 
 Solvers take no constructor arguments (solver_abc.py:26) and are driven entirely by
 keyword arguments of `solve(problem, **kwargs) -> Solution` (solver_abc.py:37-60).
-`device` is 'cuda' or 'cuda:N' (solver_abc.py:62-75); here it names a HIP device.
+`device` is 'cpu' (the default, as in the reference: the lane walk on the host CPU,
+libqbx_cpu.so), 'cuda' or 'cuda:N' (a HIP device, libqbx.so) (solver_abc.py:62-75).
 """
 from __future__ import annotations
 
@@ -11,6 +12,7 @@ from abc import ABC, abstractmethod
 
 import numpy as np
 
+from .. import _native
 from ..solution import Solution
 
 
@@ -41,18 +43,18 @@ class SolverABC(ABC):
         if "devices" in kwargs:
             raise ValueError("Use device argument")
 
-    def parse_device_arg(self, default: str = "cuda", **kwargs):
-        """'cuda' -> HIP device 0, 'cuda:N' -> device N (solver_abc.py:62-75).  This build
-        has no CPU path, so 'cpu' (the reference QBF's default) is refused."""
+    def parse_device_arg(self, default: str = "cpu", **kwargs):
+        """'cpu' (the reference's default) -> the host CPU backend, 'cuda' -> HIP device 0,
+        'cuda:N' -> device N (solver_abc.py:62-75); anything else raises ValueError."""
         device = kwargs.get("device", default)
         if not isinstance(device, str):
             raise ValueError(f"Unknown device name: {device}")
-        if device[:4] == "cuda":
+        if device == "cpu":
+            self.use_gpu = False
+            self.device_id = "cpu"
+        elif device[:4] == "cuda":
             self.use_gpu = True
-            self.device_id = int(device[5:]) if len(device) > 5 else 0
-        elif device == "cpu":
-            raise ValueError(f"{type(self).__name__} runs on the GPU (HIP) only; "
-                             "device='cpu' is not supported by qubo_lib_amd")
+            self.device_id = _native.parse_device(device)
         else:
             raise ValueError(f"Unknown device name: {device}")
         self.device_name = device

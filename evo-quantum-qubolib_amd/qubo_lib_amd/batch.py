@@ -13,7 +13,7 @@ import json
 
 import numpy as np
 
-from . import engine
+from . import _native, engine
 from .abc import ProblemABC
 from .solution import Solution
 from .state_vectors import StateVectors
@@ -57,23 +57,24 @@ def _drawable_on_device(problems) -> bool:
 
 def solve_batch(problems, device: str = "cuda", B=None, walk_bits=None, inner_bits=None,
                 draw_on_device=None) -> list:
-    """Solve every problem exactly; returns one Solution per problem, in order.
+    """Solve every problem exactly; returns one Solution per problem, in order.  On a
+    'cuda' device all problems of one arithmetic path go in one persistent launch; on
+    'cpu' they are solved one after another by the host backend.
 
     draw_on_device: seeded RandomQUBO and random TSP batches can have their matrices built
     on the GPU (bit for bit the reference's, random_qubo.py:19-33, tsp.py:50-75, 272-321)
     instead of on the host; None selects it for such batches of 64 or more problems."""
-    dev = 0
-    if isinstance(device, str) and device[:4] == "cuda":
-        dev = int(device[5:]) if len(device) > 5 else 0
-    else:
-        raise ValueError(f"solve_batch runs on the GPU only: {device}")
+    dev = _native.parse_device(device)
     family = _device_family(problems)
+    if dev == "cpu":
+        if draw_on_device:
+            raise ValueError("draw_on_device needs a 'cuda' device")
+        draw_on_device = False
     if draw_on_device and family is None:
         raise ValueError("draw_on_device needs seeded RandomQUBO or random TSP problems with one (low, high, dtype)")
     if draw_on_device is None:
         draw_on_device = family is not None and len(problems) >= 64
     if draw_on_device:
-        from . import _native
         ctx = _native.get_context(dev)
         p0 = problems[0]
         seeds = [int(p.seed) for p in problems]

@@ -11,8 +11,10 @@ Arithmetic path per problem:
   * other float Q: the FP64 walk on the float64 values of Q (the int32 walk over tables
       quantized to 2^s units filters blocks with a proven error bound; every block it
       passes is re-walked in FP64).
-  * anything else: OverflowError.  There is no CPU path: without libqbx.so or a GPU
-    every call raises.
+  * anything else: OverflowError.
+
+`device` is a HIP device (an int, 'cuda' or 'cuda:N': libqbx.so, never a CPU fallback) or
+'cpu' (libqbx_cpu.so: the same lane walk on the host, same answers).
 
 Energies of integer problems come back as Python ints, of float problems as floats.
 """
@@ -115,19 +117,19 @@ def _finish(p: Prepared, r: _native.Result) -> RawResult:
     return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=p.n, tie_B=r.tie_B, kind=p.kind, info=info)
 
 
-def solve_matrix(Q, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None) -> RawResult:
+def solve_matrix(Q, device=0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None) -> RawResult:
     """Exact minimum of x^T Q x and its argmin under the reference's tie rule.
     float_filter (float problems): None (by size), "packed" or "int32"; the answer is the same."""
     p = Q if isinstance(Q, Prepared) else prepare(Q)
     if tie_B is not None and not (0 <= int(tie_B) <= p.n):
         raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
-    ctx = _native.get_context(device)
+    ctx = _native.get_backend(device)
     kw = dict(tie_B=tie_B, walk_bits=walk_bits, inner_bits=inner_bits, float_filter=float_filter)
     r = ctx.solve_i32(p.matrix, **kw) if p.kind == "i32" else ctx.solve_f64(p.matrix, **kw)
     return _finish(p, r)
 
 
-def solve_part(Q, part: int, parts: int, device: int = 0, tie_B=None, walk_bits=None,
+def solve_part(Q, part: int, parts: int, device=0, tie_B=None, walk_bits=None,
                inner_bits=None, float_filter=None) -> RawResult:
     """The (E, rank) minimum over one of `parts` equal shares of the solve's state space (one
     GPU's share of a solve spread over `parts` GPUs); merge_parts of all shares is
@@ -136,7 +138,9 @@ def solve_part(Q, part: int, parts: int, device: int = 0, tie_B=None, walk_bits=
     p = Q if isinstance(Q, Prepared) else prepare(Q)
     if tie_B is not None and not (0 <= int(tie_B) <= p.n):
         raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
-    ctx = _native.get_context(device)
+    if _native.parse_device(device) == "cpu":
+        raise ValueError("solve_part is one GPU's share of a solve: it needs a 'cuda' device")
+    ctx = _native.get_backend(device)
     r = ctx.solve_part(p.matrix, part, parts, kind=p.kind, tie_B=tie_B, walk_bits=walk_bits,
                        inner_bits=inner_bits, float_filter=float_filter)
     return _finish(p, r)
@@ -169,12 +173,13 @@ def raw_from_native(r: _native.Result, integer: bool) -> RawResult:
     return RawResult(E=E, state=int(r.state), rank=int(r.rank), n=int(r.n), tie_B=r.tie_B, kind=kind, info=info)
 
 
-def solve_matrices(Qs, device: int = 0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None):
-    """Many problems, one persistent launch per arithmetic path.  Returns RawResults in
-    input order.  `tie_B` may be a single value (None = min(N, 20) per problem)."""
+def solve_matrices(Qs, device=0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None):
+    """Many problems, one persistent launch per arithmetic path (on 'cpu': one solve after
+    another).  Returns RawResults in input order.  `tie_B` may be a single value
+    (None = min(N, 20) per problem)."""
     preps = [q if isinstance(q, Prepared) else prepare(q) for q in Qs]
     out = [None] * len(preps)
-    ctx = _native.get_context(device)
+    ctx = _native.get_backend(device)
     for kind in ("i32", "f64"):
         idx = [i for i, p in enumerate(preps) if p.kind == kind]
         if not idx:

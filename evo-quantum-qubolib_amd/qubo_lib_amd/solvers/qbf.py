@@ -2,13 +2,15 @@
 
 `QBF().solve(problem, **kwargs) -> Solution` accepts the reference's ProblemABC objects
 (and this package's mirrors of them) and returns a Solution with the reference's fields
-(qbf.py:328-360).  The search runs on a HIP GPU (gfx950) through libqbx.so; the hot loop
-of the reference (qbf.py:188-230) and its caches (qbf.py:246-289) are replaced by the
-lane-prefix Gray walk kernels.
+(qbf.py:328-360).  The search runs on a HIP GPU (gfx950) through libqbx.so, or on the
+host CPU through libqbx_cpu.so; the hot loop of the reference (qbf.py:188-230) and its
+caches (qbf.py:246-289) are replaced by the lane-prefix walk (the gfx950 kernels, or the
+same walk compiled for the host).
 
 kwargs (qbf.py:52-59, 429-441):
-  device             'cuda' | 'cuda:N' (default 'cuda').  'cpu' raises ValueError: this
-                     build has no CPU path.
+  device             'cpu' (default, as the reference: the GPU's lane walk built for the
+                     host, OpenMP over lanes, libqbx_cpu.so) | 'cuda' | 'cuda:N' (HIP).
+                     Both give the same answers.
   A, B               the reference's prefix/suffix split.  Here it only fixes the tie rule:
                      among equal energies the answer is the state the reference QBF would
                      return with that B (first prefix in Gray order, then lowest suffix).
@@ -79,7 +81,7 @@ class QBF(SolverABC):
         self.kwargs = kwargs
         self.json_str = self.encode_kwargs(kwargs)
         self.parse_device_arg(**kwargs)
-        self.timer = PerformanceTimer(True)
+        self.timer = PerformanceTimer(self.use_gpu)
         self.N = problem.N
         self.Q = np.asarray(problem.build_Q_matrix(use_gpu=False))
         self.timer.pick_time("build_Q")

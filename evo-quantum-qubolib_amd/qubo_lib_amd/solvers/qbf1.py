@@ -35,7 +35,7 @@ class QB_F1(QBF):
         device = kwargs.get("device", "cuda")
         if not isinstance(device, str) or device[:4] != "cuda":
             raise ValueError(f"QB_F1 solver is GPU only: {device}")
-        self.parse_device_arg(**kwargs)
+        self.parse_device_arg(default="cuda", **kwargs)
         self.N = problem.N
         self.Q = np.asarray(problem.build_Q_matrix(use_gpu=False))
         if self.Q.dtype != np.int16:

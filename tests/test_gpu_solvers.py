@@ -18,7 +18,7 @@ NB = json.loads((Path(__file__).resolve().parent / "golden" / "notebook.json").r
 
 def test_qbf_solution_fields_tsp5():
     prob = TSP.init_random_instance(n_cities=5, seed=42)
-    sol = QBF().solve(prob)
+    sol = QBF().solve(prob, device="cuda")
     g = NB["tsp5_seed42"]
     assert isinstance(sol, Solution)
     assert sol.N == 25 and sol.samples_cnt == 1 and sol.solver_class == "QBF"
@@ -38,29 +38,29 @@ def test_qbf_solution_fields_tsp5():
 
 
 def test_qbf_maxcut_known_answer():
-    sol = QBF().solve(MaxCut("test"))
+    sol = QBF().solve(MaxCut("test"), device="cuda")
     assert sol.best_E == -19
 
 
 def test_qbf_tie_rule_follows_A_B_kwargs():
     Q = np.diag([-1, 0, 1, 1, 1, 1, 1, 1]).astype(np.int32)
     prob = Just_Q(Q)
-    assert "".join(map(str, QBF().solve(prob, B=0).best_X)) == "11000000"
-    assert "".join(map(str, QBF().solve(prob, A=2).best_X)) == "11000000"
-    assert "".join(map(str, QBF().solve(prob).best_X)) == "10000000"
+    assert "".join(map(str, QBF().solve(prob, B=0, device="cuda").best_X)) == "11000000"
+    assert "".join(map(str, QBF().solve(prob, A=2, device="cuda").best_X)) == "11000000"
+    assert "".join(map(str, QBF().solve(prob, device="cuda").best_X)) == "10000000"
 
 
 def test_qbf_reduced_problem_prefix_E():
     base = RandomQUBO.init_random_instance(seed=3, N=14, dtype=np.int32)
     rp = ReducedProblem(base, np.array([1, 0, 1]))
-    sol = QBF().solve(rp)
+    sol = QBF().solve(rp, device="cuda")
     assert sol.prefix_E == rp.prefix_E
     assert sol.best_E == oracle.brute_i64(rp.build_Q_matrix())[0]
 
 
 def test_qbf_energy_arr_metric():
     prob = RandomQUBO.init_random_instance(seed=4, N=12, dtype=np.int32)
-    sol = QBF().solve(prob, B=8, use_metrics="all")
+    sol = QBF().solve(prob, B=8, use_metrics="all", device="cuda")
     arr = sol.metrics[0]["energy_arr"]
     Q = prob.build_Q_matrix().astype(np.int64)
     assert arr.shape == (16,)
@@ -82,12 +82,11 @@ def test_qbf1_contract():
 
 def test_parallel_matches_qbf():
     prob = RandomQUBO.init_random_instance(seed=9, N=22, dtype=np.int32)
-    whole = QBF().solve(prob)
+    whole = QBF().solve(prob, device="cuda")
     for splits in (1, 2, 8):
         sol = Parallel(QBF, splits, ["cuda"]).solve(prob)
         assert sol.best_E == whole.best_E
-        x = sol.best_X.astype(np.int64)
-        assert x @ prob.build_Q_matrix().astype(np.int64) @ x == whole.best_E
+        assert np.array_equal(sol.best_X, whole.best_X)
         assert sol.epochs_cnt == splits and len(sol.worker_logs) == splits
     sol = Parallel(QB_F1, 4, ["cuda:0"]).solve(RandomQUBO.init_random_instance(seed=9, N=20, dtype=np.int16))
     assert sol.solver_class == "QB_F1"
@@ -98,5 +97,5 @@ def test_solve_batch_matches_qbf():
     probs += [TSP.init_random_instance(n_cities=4, seed=s) for s in range(3)]
     sols = solve_batch(probs)
     for p, s in zip(probs, sols):
-        one = QBF().solve(p)
+        one = QBF().solve(p, device="cuda")
         assert s.best_E == one.best_E and np.array_equal(s.best_X, one.best_X)

@@ -59,10 +59,10 @@ def test_solver_argument_errors():
         QBF().solve(prob, gpu_id=0)
     with pytest.raises(ValueError, match="device"):
         QBF().solve(prob, devices=["cuda"])
-    with pytest.raises(ValueError, match="GPU"):
-        QBF().solve(prob, device="cpu")
     with pytest.raises(ValueError, match="Unknown device"):
         QBF().solve(prob, device="tpu")
+    with pytest.raises(ValueError, match="Unknown device"):
+        QBF().solve(prob, device="cuda:x")
     with pytest.raises(ValueError, match="GPU only"):
         QB_F1().solve(prob, device="cpu")
     with pytest.raises(TypeError):
@@ -78,8 +78,11 @@ def test_parallel_validation():
         Parallel(QBF, 2, [])
     with pytest.raises(ValueError, match="mix"):
         Parallel(QBF, 2, ["cpu", "cuda"])
-    with pytest.raises(ValueError, match="GPUs only"):
-        Parallel(QBF, 2, ["cpu"] * 2)
+    with pytest.raises(ValueError, match="GPU only"):
+        Parallel(QB_F1, 2, ["cpu"])
+    with pytest.raises(ValueError, match="Unknown device"):
+        Parallel(QBF, 2, ["tpu"])
+    assert Parallel(QBF, 2, ["cpu"] * 2)._devices == ["cpu", "cpu"]
     with pytest.raises(ValueError, match="Unsupported"):
         Parallel(object, 2, ["cuda"])
     p = Parallel(QBF, 4, ["cuda"] * 2)

@@ -9,10 +9,11 @@ import pytest
 from qubo_lib_amd import _native
 
 HEADER = Path(__file__).resolve().parents[1] / "include" / "qbx.h"
+CPU_HEADER = HEADER.with_name("qbx_cpu.h")
 
 
-def declared_functions():
-    text = HEADER.read_text()
+def declared_functions(header=HEADER):
+    text = header.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(qbx_[a-z0-9_]+)\s*\(", text)))
 
@@ -24,6 +25,13 @@ def test_header_and_binding_agree():
 def test_library_exports_every_declared_symbol():
     lib = _native.load_library()
     for name in declared_functions():
+        assert isinstance(getattr(lib, name), ctypes._CFuncPtr), name
+
+
+def test_cpu_library_exports_every_declared_symbol():
+    assert declared_functions(CPU_HEADER) == sorted(_native.CPU_EXPORTED_SYMBOLS)
+    lib = _native.load_cpu_library()
+    for name in declared_functions(CPU_HEADER):
         assert isinstance(getattr(lib, name), ctypes._CFuncPtr), name
 
 
@@ -43,7 +51,7 @@ def test_no_cpu_fallback_without_gpu():
     from qubo_lib_amd.solvers import QBF
     from qubo_lib_amd.problems import RandomQUBO
     with pytest.raises(_native.NativeError):
-        QBF().solve(RandomQUBO.init_random_instance(seed=0, N=8, dtype="int32"))
+        QBF().solve(RandomQUBO.init_random_instance(seed=0, N=8, dtype="int32"), device="cuda")
 
 
 def test_path_codes_match_header():
