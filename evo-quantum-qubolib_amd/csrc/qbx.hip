@@ -66,6 +66,11 @@ struct Layout {
     int gen_dtype = 0;
     int64_t gen_low = 0, gen_high = 0;        // RandomQUBO bounds
     size_t off_rng = 0;
+    // per-reference-prefix records of a single solve (qbx_solve_minima_*)
+    bool minima = false;
+    uint64_t nprefix = 0;                      // 2^(n - B)
+    size_t off_minima = 0, off_ticks = 0;      // device sections, nprefix x 8 B each
+    size_t h_minima = 0;                       // host copy: minima then ticks, after the results
 };
 }  // namespace qbx
 
@@ -107,7 +112,7 @@ int ensure_capacity(qbx_ctx* c, size_t dbytes, size_t hbytes) {
 }
 
 int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_opts_t* opts, bool single,
-                Layout& lay, size_t gen_rec = 0) {
+                Layout& lay, size_t gen_rec = 0, bool minima = false) {
     if (count <= 0) return set_err(QBX_EINVAL, "count must be positive, got %d", count);
     if (!ns) return set_err(QBX_EINVAL, "null argument");
     qbx_opts_t o = {-1, -1, -1, 0};
@@ -134,7 +139,7 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     const PlanMode mode = single ? PLAN_SINGLE : PLAN_BATCH;
     for (int i = 0; i < count; ++i) {
         Plan& p = plans[i];
-        const int rc = plan_problem(kind, ns[i], o, mode, i, p, &c->walk);
+        const int rc = plan_problem(kind, ns[i], o, mode, i, p, &c->walk, minima);
         if (rc) return rc;
         p.off_Q = off;
         off = align256(off + lay.esize * (size_t)p.n * p.n);
@@ -165,6 +170,20 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     off = align256(off + sizeof(ResultDev) * count);
     lay.off_ticket = off;
     off = align256(off + sizeof(unsigned long long) * MAX_GROUPS);
+    if (minima) {
+        const int A = plans[0].n - plans[0].B;
+        if (count != 1) return set_err(QBX_EINVAL, "per-prefix records are kept for single solves");
+        if (A > QBX_MAX_PREFIX_BITS)
+            return set_err(QBX_EINVAL, "per-prefix records of 2^%d prefixes: at most 2^%d (raise tie_B)", A,
+                           QBX_MAX_PREFIX_BITS);
+        lay.minima = true;
+        lay.nprefix = 1ull << A;
+        lay.off_minima = off;
+        off = align256(off + 8 * (size_t)lay.nprefix);
+        lay.off_ticks = off;
+        off = align256(off + 8 * (size_t)lay.nprefix);
+        lay.h_minima = align256(sizeof(ResultDev) * (size_t)count);
+    }
     lay.total_bytes = off;
 
     lay.order.resize(count);
@@ -205,6 +224,11 @@ void stage_descriptors(const Layout& lay, char* h, char* dv) {
         I.cand = (uint32_t*)(dv + p.off_cand);
         I.slots = (SlotDev*)(dv + p.off_slots);
         I.result = (ResultDev*)(dv + lay.off_res) + i;
+        if (lay.minima) {
+            I.minima = (uint64_t*)(dv + lay.off_minima);
+            I.ticks = (uint64_t*)(dv + lay.off_ticks);
+            I.mshift = p.B - p.L;
+        }
     }
     TileRange* hr = (TileRange*)(h + lay.off_ranges);
     // tickets count from 0 within each variant's launch
@@ -256,6 +280,14 @@ int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const c
     if (lay.gen) {
         HIP_TRY(launch_gen(lay, dv, stream));
     }
+    if (lay.minima) {   // fill kernels, not memsets: the enqueued work stays graph-safe
+        const unsigned blocks = (unsigned)std::min<uint64_t>(4096, (lay.nprefix + 255) / 256);
+        hipLaunchKernelGGL(qbx_fill_u64, dim3(blocks), dim3(256), 0, stream, (uint64_t*)(dv + lay.off_minima),
+                           (uint64_t)lay.nprefix, ~0ull);
+        hipLaunchKernelGGL(qbx_fill_u64, dim3(blocks), dim3(256), 0, stream, (uint64_t*)(dv + lay.off_ticks),
+                           (uint64_t)lay.nprefix, 0ull);
+        HIP_TRY(hipGetLastError());
+    }
     unsigned long long* dticket = (unsigned long long*)(dv + lay.off_ticket);
     hipLaunchKernelGGL(qbx_prep, dim3(lay.count), dim3(256), 0, stream, (InstDev*)(dv + lay.off_inst),
                        (const uint32_t*)nullptr, dticket, (const InstDev*)nullptr);
@@ -273,6 +305,10 @@ int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const c
     }
     if (ev_end) HIP_TRY(hipEventRecord(ev_end, stream));
     HIP_TRY(hipMemcpyAsync(h_res, dv + lay.off_res, sizeof(ResultDev) * lay.count, hipMemcpyDeviceToHost, stream));
+    if (lay.minima) {   // minima and ticks are adjacent sections: one copy
+        HIP_TRY(hipMemcpyAsync(h_res + lay.h_minima, dv + lay.off_minima, lay.off_ticks - lay.off_minima +
+                               8 * (size_t)lay.nprefix, hipMemcpyDeviceToHost, stream));
+    }
     return QBX_OK;
 }
 
@@ -308,14 +344,39 @@ int collect(const Layout& lay, const ResultDev* hres, float ms, qbx_result_t* ou
     return first_bad;
 }
 
+// Per-prefix outputs of qbx_solve_minima_*: minima as int64 (integer kind) or double.
+struct MinimaOut {
+    void* minima;
+    double* wave_ms;    // nullable
+};
+
+// Keys and ticks of the host copy into the caller's arrays.
+void write_minima(const qbx_ctx* c, const Layout& lay, const char* h, const MinimaOut& mo) {
+    const uint64_t* keys = (const uint64_t*)(h + lay.h_minima);
+    const uint64_t* ticks = (const uint64_t*)(h + lay.h_minima + (lay.off_ticks - lay.off_minima));
+    for (uint64_t i = 0; i < lay.nprefix; ++i) {
+        if (lay.kind == KIND_I32) {
+            ((int64_t*)mo.minima)[i] = (int64_t)(keys[i] ^ 0x8000000000000000ull);
+        } else {
+            const uint64_t k = keys[i];
+            const uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+            double e;
+            memcpy(&e, &u, sizeof e);
+            ((double*)mo.minima)[i] = e;
+        }
+        if (mo.wave_ms) mo.wave_ms[i] = c->wall_khz > 0 ? (double)ticks[i] / 64.0 / (double)c->wall_khz : 0.0;
+    }
+}
+
 template <class T>
 int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* offsets,
               const int32_t* ns, const qbx_opts_t* opts, qbx_result_t* out, bool single,
-              int part = 0, int parts = 1) {
+              int part = 0, int parts = 1, const MinimaOut* mo = nullptr) {
     if (!c) return set_err(QBX_EINVAL, "null context");
     if (!Qpacked || !offsets || !ns || !out) return set_err(QBX_EINVAL, "null argument");
+    if (mo && !mo->minima) return set_err(QBX_EINVAL, "null argument");
     Layout lay;
-    int rc = make_layout(c, kind, count, ns, opts, single, lay);
+    int rc = make_layout(c, kind, count, ns, opts, single, lay, 0, mo != nullptr);
     if (rc) return rc;
     if (parts != 1) {   // one contiguous range of the (single) problem's chunks
         const uint32_t chunks = lay.plans[0].chunks;
@@ -327,7 +388,9 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
         lay.part_count = e - b;
         lay.groups[0].total = e - b;
     }
-    rc = ensure_capacity(c, lay.total_bytes, std::max(lay.stage_bytes, sizeof(ResultDev) * (size_t)count));
+    size_t hneed = std::max(lay.stage_bytes, sizeof(ResultDev) * (size_t)count);
+    if (lay.minima) hneed = std::max(hneed, lay.h_minima + 16 * (size_t)lay.nprefix);
+    rc = ensure_capacity(c, lay.total_bytes, hneed);
     if (rc) return rc;
     char* h = c->hbuf;
     char* dv = c->dbuf;
@@ -341,6 +404,7 @@ int run_batch(qbx_ctx* c, int kind, int count, const T* Qpacked, const int64_t* 
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     rc = collect(lay, (const ResultDev*)h, ms, out);
+    if (mo && rc == QBX_OK) write_minima(c, lay, h, *mo);
     if (lay.part_count) {   // states of this part only
         const Plan& p = lay.plans[0];
         const uint64_t t0 = (uint64_t)lay.part_begin * p.tpc;
@@ -490,6 +554,7 @@ int qbx_open(int device, qbx_ctx_t** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device);
     for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
         int b = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kVariants[v].fn, 64, 0);
@@ -540,6 +605,22 @@ int qbx_solve_f64(qbx_ctx_t* c, const double* Q, int n, const qbx_opts_t* opts, 
     const int64_t off = 0;
     const int32_t nn = n;
     return run_batch<double>(c, KIND_F64, 1, Q, &off, &nn, opts, out, true);
+}
+
+int qbx_solve_minima_i32(qbx_ctx_t* c, const int32_t* Q, int n, const qbx_opts_t* opts, int64_t* minima,
+                         double* wave_ms, qbx_result_t* out) {
+    const int64_t off = 0;
+    const int32_t nn = n;
+    const MinimaOut mo{minima, wave_ms};
+    return run_batch<int32_t>(c, KIND_I32, 1, Q, &off, &nn, opts, out, true, 0, 1, &mo);
+}
+
+int qbx_solve_minima_f64(qbx_ctx_t* c, const double* Q, int n, const qbx_opts_t* opts, double* minima,
+                         double* wave_ms, qbx_result_t* out) {
+    const int64_t off = 0;
+    const int32_t nn = n;
+    const MinimaOut mo{minima, wave_ms};
+    return run_batch<double>(c, KIND_F64, 1, Q, &off, &nn, opts, out, true, 0, 1, &mo);
 }
 
 int qbx_set_walk_table(qbx_ctx_t* c, int kind, int mode, const int32_t* walk_bits, int count) {

@@ -55,7 +55,9 @@ struct Tiling {
     int n, B, M, OMAX, L, H;
 };
 
-int plan(int n, const qbx_opts_t* opts, Tiling& t) {
+// cap_L_at_B: lanes no wider than the tie width B, so that every lane lies inside one
+// reference prefix (per-prefix records; needs B >= 1).
+int plan(int n, const qbx_opts_t* opts, Tiling& t, bool cap_L_at_B = false) {
     qbx_opts_t o = {-1, -1, -1, 0};
     if (opts) o = *opts;
     if (n < 1 || n > MAX_N) return set_err(QBX_EINVAL, "n=%d outside [1, %d]", n, MAX_N);
@@ -64,7 +66,12 @@ int plan(int n, const qbx_opts_t* opts, Tiling& t) {
     t.n = n;
     t.B = o.tie_B < 0 ? (n < 20 ? n : 20) : o.tie_B;
     if (t.B > n) return set_err(QBX_EINVAL, "tie_B=%d exceeds n=%d", t.B, n);
-    const int want = o.inner_bits < 0 ? 7 : o.inner_bits;
+    if (cap_L_at_B && t.B < 1) return set_err(QBX_EINVAL, "per-prefix records need tie_B >= 1");
+    if (cap_L_at_B && n - t.B > QBX_MAX_PREFIX_BITS)
+        return set_err(QBX_EINVAL, "per-prefix records of 2^%d prefixes: at most 2^%d (raise tie_B)", n - t.B,
+                       QBX_MAX_PREFIX_BITS);
+    int want = o.inner_bits < 0 ? 7 : o.inner_bits;
+    if (cap_L_at_B && want > t.B) want = t.B;
     static const int kM[4] = {7, 6, 4, 1}, kO[4] = {13, 14, 16, 19};
     t.M = -1;
     for (int v = 0; v < 4; ++v)
@@ -80,6 +87,7 @@ int plan(int n, const qbx_opts_t* opts, Tiling& t) {
         if (L > n) L = n;
     }
     if (L < t.M) L = t.M;
+    if (cap_L_at_B && L > t.B) L = t.B;
     if (L > t.M + t.OMAX || L > n)
         return set_err(QBX_EINVAL, "walk_bits=%d outside [%d, %d]", L, t.M, n < t.M + t.OMAX ? n : t.M + t.OMAX);
     t.L = L;
@@ -104,9 +112,36 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Per-reference-prefix records (qbx_cpu_solve_minima_*): lane lambda lies in prefix
+// lambda >> (B - L); each prefix keeps the order-preserving key of its least energy and
+// the nanoseconds its lanes took.
+struct Prefixes {
+    std::vector<uint64_t> key, ns;
+    int shift = 0;
+    bool on = false;
+    void init(const Tiling& t) {
+        on = true;
+        shift = t.B - t.L;
+        key.assign((size_t)1 << (t.n - t.B), ~0ull);
+        ns.assign((size_t)1 << (t.n - t.B), 0ull);
+    }
+    void add(uint64_t lam, uint64_t k, uint64_t dns) {
+        uint64_t* slot = &key[lam >> shift];
+        uint64_t cur = __atomic_load_n(slot, __ATOMIC_RELAXED);
+        while (k < cur && !__atomic_compare_exchange_n(slot, &cur, k, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+        __atomic_fetch_add(&ns[lam >> shift], dns, __ATOMIC_RELAXED);
+    }
+};
+
+uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 // ------------------------------------------------------------------------ integer Q
 template <int M, int OMAX>
-int solve_i32_t(const int32_t* Q, const Tiling& t, int threads, qbx_result_t* out) {
+int solve_i32_t(const int32_t* Q, const Tiling& t, int threads, qbx_result_t* out, Prefixes* pre) {
     const int n = t.n, L = t.L, B = t.B;
     std::vector<int32_t> W((size_t)n * LDW, 0), d(64, 0), dl(1u << M, 0);
     long long bound = 0;
@@ -137,9 +172,11 @@ int solve_i32_t(const int32_t* Q, const Tiling& t, int threads, qbx_result_t* ou
         uint32_t K = 0xffffffffu;
 #pragma omp for schedule(dynamic, 16) nowait
         for (long long lam = 0; lam < lanes; ++lam) {
+            const uint64_t c0 = pre ? now_ns() : 0;
             const int64_t key = lane_best_block_i32<M, OMAX>(W.data(), d.data(), n, L, B, (uint64_t)lam, dl.data());
             // lanes of one thread come in increasing order: a strict test keeps the lowest
             if (key_E(key) < E) { E = key_E(key); lamE = (uint64_t)lam; K = key_K(key); }
+            if (pre) pre->add((uint64_t)lam, key_of_i64(key_E(key)), now_ns() - c0);
         }
 #pragma omp critical(qbx_cpu_reduce)
         {
@@ -169,9 +206,24 @@ struct AtomicBound {
     }
 };
 
+// No shared bound: every lane finds its own exact minimum (per-prefix records).
+struct NoBound {
+    double refresh() const { return __builtin_inf(); }
+    void publish(double) {}
+};
+
+template <int M, int OMAX, class Bound>
+LaneOutF64 walk_f64_lane(const std::vector<int32_t>& Wq, const std::vector<int32_t>& dq, const std::vector<double>& W,
+                         const std::vector<double>& d, const Tiling& t, uint64_t lam, const std::vector<int32_t>& dlq,
+                         const std::vector<double>& dld, double scale, double qm, Bound& src, uint32_t* bc,
+                         int32_t* bm) {
+    return walk_lane_q32<M, OMAX>(Wq.data(), dq.data(), W.data(), d.data(), t.n, t.L, t.B, lam, dlq.data(),
+                                  dld.data(), scale, qm, src, bc, bm, 1);
+}
+
 template <int M, int OMAX>
-int solve_f64_t(const double* Q, const Tiling& t, int threads, qbx_result_t* out) {
-    const int n = t.n, L = t.L, B = t.B;
+int solve_f64_t(const double* Q, const Tiling& t, int threads, qbx_result_t* out, Prefixes* pre) {
+    const int n = t.n, B = t.B;
     std::vector<double> W((size_t)n * LDW, 0.0), d(64, 0.0), dld(1u << M, 0.0);
     std::vector<int32_t> Wq((size_t)n * LDW, 0), dq(64, 0), dlq(1u << M, 0);
     double bound = 0.0;
@@ -215,12 +267,15 @@ int solve_f64_t(const double* Q, const Tiling& t, int threads, qbx_result_t* out
         int32_t buf_m[QCAP];
         double E = __builtin_inf();
         uint64_t R = ~0ull;
+        NoBound none;
 #pragma omp for schedule(dynamic, 16) nowait
         for (long long lam = 0; lam < lanes; ++lam) {
-            const LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq.data(), dq.data(), W.data(), d.data(), n, L, B,
-                                                        (uint64_t)lam, dlq.data(), dld.data(), scale, qm, src,
-                                                        buf_c, buf_m, 1);
+            const uint64_t c0 = pre ? now_ns() : 0;
+            const LaneOutF64 o =
+                pre ? walk_f64_lane<M, OMAX>(Wq, dq, W, d, t, (uint64_t)lam, dlq, dld, scale, qm, none, buf_c, buf_m)
+                    : walk_f64_lane<M, OMAX>(Wq, dq, W, d, t, (uint64_t)lam, dlq, dld, scale, qm, src, buf_c, buf_m);
             if (o.E < E || (o.E == E && o.rank < R)) { E = o.E; R = o.rank; }
+            if (pre) pre->add((uint64_t)lam, key_of_f64(o.E), now_ns() - c0);
         }
 #pragma omp critical(qbx_cpu_reduce_f)
         {
@@ -235,6 +290,39 @@ int solve_f64_t(const double* Q, const Tiling& t, int threads, qbx_result_t* out
     return QBX_OK;
 }
 
+int run_i32(const int32_t* Q, const Tiling& t, int threads, qbx_result_t* out, Prefixes* pre) {
+    switch (t.M) {
+        case 7: return solve_i32_t<7, 13>(Q, t, threads, out, pre);
+        case 6: return solve_i32_t<6, 14>(Q, t, threads, out, pre);
+        case 4: return solve_i32_t<4, 16>(Q, t, threads, out, pre);
+        default: return solve_i32_t<1, 19>(Q, t, threads, out, pre);
+    }
+}
+
+int run_f64(const double* Q, const Tiling& t, int threads, qbx_result_t* out, Prefixes* pre) {
+    switch (t.M) {
+        case 7: return solve_f64_t<7, 13>(Q, t, threads, out, pre);
+        case 6: return solve_f64_t<6, 14>(Q, t, threads, out, pre);
+        case 4: return solve_f64_t<4, 16>(Q, t, threads, out, pre);
+        default: return solve_f64_t<1, 19>(Q, t, threads, out, pre);
+    }
+}
+
+void write_prefixes(const Prefixes& pre, bool f64, void* minima, double* wave_ms) {
+    for (size_t i = 0; i < pre.key.size(); ++i) {
+        const uint64_t k = pre.key[i];
+        if (f64) {
+            const uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+            double e;
+            memcpy(&e, &u, sizeof e);
+            ((double*)minima)[i] = e;
+        } else {
+            ((int64_t*)minima)[i] = (int64_t)(k ^ 0x8000000000000000ull);
+        }
+        if (wave_ms) wave_ms[i] = (double)pre.ns[i] * 1e-6;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -245,26 +333,40 @@ int qbx_cpu_solve_i32(const int32_t* Q, int n, const qbx_opts_t* opts, int threa
     if (!Q || !out) return set_err(QBX_EINVAL, "null argument");
     Tiling t;
     const int rc = plan(n, opts, t);
-    if (rc) return rc;
-    switch (t.M) {
-        case 7: return solve_i32_t<7, 13>(Q, t, threads, out);
-        case 6: return solve_i32_t<6, 14>(Q, t, threads, out);
-        case 4: return solve_i32_t<4, 16>(Q, t, threads, out);
-        default: return solve_i32_t<1, 19>(Q, t, threads, out);
-    }
+    return rc ? rc : run_i32(Q, t, threads, out, nullptr);
 }
 
 int qbx_cpu_solve_f64(const double* Q, int n, const qbx_opts_t* opts, int threads, qbx_result_t* out) {
     if (!Q || !out) return set_err(QBX_EINVAL, "null argument");
     Tiling t;
     const int rc = plan(n, opts, t);
+    return rc ? rc : run_f64(Q, t, threads, out, nullptr);
+}
+
+int qbx_cpu_solve_minima_i32(const int32_t* Q, int n, const qbx_opts_t* opts, int threads, int64_t* minima,
+                             double* wave_ms, qbx_result_t* out) {
+    if (!Q || !out || !minima) return set_err(QBX_EINVAL, "null argument");
+    Tiling t;
+    int rc = plan(n, opts, t, true);
     if (rc) return rc;
-    switch (t.M) {
-        case 7: return solve_f64_t<7, 13>(Q, t, threads, out);
-        case 6: return solve_f64_t<6, 14>(Q, t, threads, out);
-        case 4: return solve_f64_t<4, 16>(Q, t, threads, out);
-        default: return solve_f64_t<1, 19>(Q, t, threads, out);
-    }
+    Prefixes pre;
+    pre.init(t);
+    rc = run_i32(Q, t, threads, out, &pre);
+    if (rc == QBX_OK) write_prefixes(pre, false, minima, wave_ms);
+    return rc;
+}
+
+int qbx_cpu_solve_minima_f64(const double* Q, int n, const qbx_opts_t* opts, int threads, double* minima,
+                             double* wave_ms, qbx_result_t* out) {
+    if (!Q || !out || !minima) return set_err(QBX_EINVAL, "null argument");
+    Tiling t;
+    int rc = plan(n, opts, t, true);
+    if (rc) return rc;
+    Prefixes pre;
+    pre.init(t);
+    rc = run_f64(Q, t, threads, out, &pre);
+    if (rc == QBX_OK) write_prefixes(pre, true, minima, wave_ms);
+    return rc;
 }
 
 }  // extern "C"

@@ -59,6 +59,13 @@ struct InstDev {
     int32_t slack;           // rebased pair walks: PAIR_BOUND - D, the guard's room
     int32_t qshift;          // packed float filter: coarse scale 2^-qshift of the quantized one
     int32_t flags;           // qbx_opts_t.flags (QBX_FLAG_FLOAT_INT32_FILTER)
+    // per-reference-prefix records (nullable; single solves with L <= B): the prefix of rank
+    // r is r >> B, so lane lambda (ranks lambda 2^L ..) lies in prefix lambda >> mshift
+    uint64_t* minima;        // order-preserving key of each prefix's least energy (starts ~0)
+    uint64_t* ticks;         // wall-clock ticks x 64 of the waves that walked each prefix, prorated
+                             // by lanes (starts 0)
+    int32_t mshift;          // B - L: lanes per prefix = 2^mshift
+    int32_t pad_;
 };
 
 // Table regions of one problem (bytes), shared by the batch and serving layouts.
@@ -274,6 +281,30 @@ static __global__ __launch_bounds__(64) void qbx_gen_tsp(const InstDev* insts, i
     else tsp_qubo<double>(nc, d, (double*)I.Q);
 }
 
+// Grid-stride fill of a device buffer (graph-safe, where a memset node is not).
+static __global__ __launch_bounds__(256) void qbx_fill_u64(uint64_t* p, uint64_t count, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256u) p[i] = v;
+}
+
+// Per-reference-prefix records of one tile (all 64 lanes call it together): key is the
+// order-preserving key of the lane's least energy (~0 for a lane past the problem's end),
+// dt the wave's wall-clock ticks for the tile.  Each aligned run of min(2^mshift, 64) lanes
+// belongs to one prefix: its minimum key is folded into the prefix's entry, and the run's
+// share of the tile's time (its lanes / 64 of dt) into the prefix's ticks (kept x 64).
+__device__ __forceinline__ void record_prefixes(const InstDev& I, uint64_t lam0, uint64_t key, uint64_t dt,
+                                                int lane) {
+    const int gw = I.mshift < 6 ? I.mshift : 6;
+    for (int s = 0; s < gw; ++s) {
+        const uint64_t k2 = __shfl_xor(key, 1 << s);
+        key = k2 < key ? k2 : key;
+    }
+    if ((lane & ((1 << gw) - 1)) == 0 && key != ~0ull) {
+        const uint64_t p = (lam0 + (uint64_t)lane) >> I.mshift;
+        atomicMin((unsigned long long*)(I.minima + p), (unsigned long long)key);
+        atomicAdd((unsigned long long*)(I.ticks + p), (unsigned long long)(dt << gw));
+    }
+}
+
 // ------------------------------------------------------------- tile commit / finalize
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -386,7 +417,10 @@ __device__ __forceinline__ void wave_best_block(int32_t& Eb, uint64_t& lamb, uin
 // The walk qbx_prep chose per problem: the packed pair walk (bound <= PAIR_BOUND), the
 // rebased pair walk (block-relative 16-bit energies, guarded per group), or the int32 walk.
 // All give the same lane keys; the winning block is re-walked by the int32 code either way.
-template <int M, int OMAX>
+// REC: per-reference-prefix records (I.minima).  A template flag, so that the walk without
+// records compiles exactly as if they did not exist (its kernel is at the scalar-register
+// ceiling, and a timestamp kept live across the walk costs it 3%).
+template <int M, int OMAX, bool REC>
 static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
                                                    unsigned long long total,
@@ -407,6 +441,7 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
         const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
         uint64_t rank = ~0ull;
         const int path = I.path;
+        constexpr bool rec = REC;
         if (I.status == 0 && (path == QBX_PATH_PAIR16 || path == QBX_PATH_PAIR16_REBASED)) {
             const uint32_t* Wp = (const uint32_t*)(W + (size_t)n * LDW);
             // packed trie constants: uniform values kept in VGPRs (the lane-dependent zero
@@ -417,13 +452,17 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
             for (int k = 0; k < (1 << M); ++k) dlp[k] = psrc[k + (lane & zero)];
             for (uint32_t tile = t0; tile < t1; ++tile) {
                 const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
+                const uint64_t c0 = rec ? wall_clock64() : 0ull;
+                uint64_t lkey = ~0ull;
                 if (H >= 6 || lam < (1ull << H)) {
                     const int64_t key = path == QBX_PATH_PAIR16
                                             ? lane_best_block_pair<M, OMAX>(W, Wp, d, n, L, B, lam, dlp)
                                             : lane_best_block_rp<M, OMAX>(W, Wp, d, n, L, B, lam, dlp, I.slack,
                                                                           nullptr);
                     if (key_E(key) < Eb) { Eb = key_E(key); lamb = lam; Kb = key_K(key); }
+                    lkey = key_of_i64(key_E(key));
                 }
+                if (rec) record_prefixes(I, (uint64_t)tile * 64u, lkey, wall_clock64() - c0, lane);
             }
             wave_best_block(Eb, lamb, Kb);
             // the wave's winning block: one int32 re-walk for its lowest inner rank, with the
@@ -437,10 +476,14 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
             if (I.status == 0) {
                 for (uint32_t tile = t0; tile < t1; ++tile) {
                     const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
+                    const uint64_t c0 = rec ? wall_clock64() : 0ull;
+                    uint64_t lkey = ~0ull;
                     if (H >= 6 || lam < (1ull << H)) {
                         const int64_t key = lane_best_block_i32<M, OMAX>(W, d, n, L, B, lam, dl);
                         if (key_E(key) < Eb) { Eb = key_E(key); lamb = lam; Kb = key_K(key); }
+                        lkey = key_of_i64(key_E(key));
                     }
+                    if (rec) record_prefixes(I, (uint64_t)tile * 64u, lkey, wall_clock64() - c0, lane);
                 }
             }
             wave_best_block(Eb, lamb, Kb);
@@ -460,8 +503,12 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
 struct WaveBound {
     unsigned long long* minkey;
     double offered;
-    __device__ double refresh() const { return energy_of_key(ld_relaxed((const uint64_t*)minkey)); }
+    bool live;     // false: no shared bound, every lane finds its own exact minimum
+    __device__ double refresh() const {
+        return live ? energy_of_key(ld_relaxed((const uint64_t*)minkey)) : __builtin_inf();
+    }
     __device__ void publish(double e) {
+        if (!live) return;
         for (int off = 32; off > 0; off >>= 1) {
             const double e2 = __shfl_xor(e, off);
             e = e2 < e ? e2 : e;
@@ -474,7 +521,7 @@ struct WaveBound {
 };
 
 // Two waves per SIMD (at most 256 VGPRs): the filters' latency hides behind the other wave.
-template <int M, int OMAX>
+template <int M, int OMAX, bool REC>
 static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void qbx_walk_f64(const InstDev* __restrict__ insts,
                                                    const TileRange* __restrict__ ranges, int nranges,
                                                    unsigned long long total,
@@ -495,8 +542,10 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
             const int32_t* dsrc = (const int32_t*)(dld + (1 << M));
             const uint32_t t0 = chunk * I.tpc;
             const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
-            // branch and bound across waves through the problem's running minimum key
-            WaveBound wb{&I.red->minkey, __builtin_inf()};
+            // branch and bound across waves through the problem's running minimum key; with
+            // per-prefix records every lane needs its own exact minimum, so no shared bound
+            constexpr bool rec = REC;
+            WaveBound wb{&I.red->minkey, __builtin_inf(), !rec};
             // per-lane candidate records, lane-interleaved (entry k of lane l at k * 64 + l)
             __shared__ uint32_t qc[QCAP * 64];
             __shared__ int32_t qmn[QCAP * 64];
@@ -512,10 +561,12 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                     const uint64_t lam0 = (uint64_t)tile * 64u;
                     const uint64_t lam = lam0 + (uint64_t)lane;
                     const bool active = H >= 6 || lam < (1ull << H);
+                    const uint64_t c0 = rec ? wall_clock64() : 0ull;
                     const LaneOutF64 o = walk_lane_qrp<M, OMAX>(Wq, dq, Wpc, W, d, n, L, I.B, active ? lam : lam0,
                                                                 dlp, dsrc, dld, I.qscale, I.qmargin, I.qshift,
                                                                 I.slack, wb, qc + lane, qmn + lane, 64, nullptr);
                     if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
+                    if (rec) record_prefixes(I, lam0, active ? key_of_f64(o.E) : ~0ull, wall_clock64() - c0, lane);
                 }
             } else {
                 int32_t dlq[1 << M];
@@ -525,10 +576,12 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                     const uint64_t lam0 = (uint64_t)tile * 64u;
                     const uint64_t lam = lam0 + (uint64_t)lane;
                     const bool active = H >= 6 || lam < (1ull << H);
+                    const uint64_t c0 = rec ? wall_clock64() : 0ull;
                     const LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq, dq, W, d, n, L, I.B, active ? lam : lam0, dlq,
                                                                 dld, I.qscale, I.qmargin, wb, qc + lane, qmn + lane,
                                                                 64);
                     if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
+                    if (rec) record_prefixes(I, lam0, active ? key_of_f64(o.E) : ~0ull, wall_clock64() - c0, lane);
                 }
             }
         }

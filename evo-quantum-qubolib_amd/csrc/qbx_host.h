@@ -24,20 +24,29 @@ typedef void (*walk_fn_t)(const InstDev*, const TileRange*, int, unsigned long l
 
 struct Variant {
     int kind, M, OMAX;
+    bool rec;          // records per-reference-prefix minima (single solves of qbx_solve_minima_*)
     const void* fn;
 };
 
-// available (kind, M) instantiations, largest M first within each kind (each translation
-// unit launches its own copies of the static kernels)
+// available (kind, M, rec) instantiations, largest M first within each kind (each
+// translation unit launches its own copies of the static kernels)
 static const Variant kVariants[] = {
-    {KIND_I32, 7, 13, (const void*)qbx_walk_i32<7, 13>},
-    {KIND_I32, 6, 14, (const void*)qbx_walk_i32<6, 14>},
-    {KIND_I32, 4, 16, (const void*)qbx_walk_i32<4, 16>},
-    {KIND_I32, 1, 19, (const void*)qbx_walk_i32<1, 19>},
-    {KIND_F64, 7, 13, (const void*)qbx_walk_f64<7, 13>},
-    {KIND_F64, 6, 14, (const void*)qbx_walk_f64<6, 14>},
-    {KIND_F64, 4, 16, (const void*)qbx_walk_f64<4, 16>},
-    {KIND_F64, 1, 19, (const void*)qbx_walk_f64<1, 19>},
+    {KIND_I32, 7, 13, false, (const void*)qbx_walk_i32<7, 13, false>},
+    {KIND_I32, 6, 14, false, (const void*)qbx_walk_i32<6, 14, false>},
+    {KIND_I32, 4, 16, false, (const void*)qbx_walk_i32<4, 16, false>},
+    {KIND_I32, 1, 19, false, (const void*)qbx_walk_i32<1, 19, false>},
+    {KIND_F64, 7, 13, false, (const void*)qbx_walk_f64<7, 13, false>},
+    {KIND_F64, 6, 14, false, (const void*)qbx_walk_f64<6, 14, false>},
+    {KIND_F64, 4, 16, false, (const void*)qbx_walk_f64<4, 16, false>},
+    {KIND_F64, 1, 19, false, (const void*)qbx_walk_f64<1, 19, false>},
+    {KIND_I32, 7, 13, true, (const void*)qbx_walk_i32<7, 13, true>},
+    {KIND_I32, 6, 14, true, (const void*)qbx_walk_i32<6, 14, true>},
+    {KIND_I32, 4, 16, true, (const void*)qbx_walk_i32<4, 16, true>},
+    {KIND_I32, 1, 19, true, (const void*)qbx_walk_i32<1, 19, true>},
+    {KIND_F64, 7, 13, true, (const void*)qbx_walk_f64<7, 13, true>},
+    {KIND_F64, 6, 14, true, (const void*)qbx_walk_f64<6, 14, true>},
+    {KIND_F64, 4, 16, true, (const void*)qbx_walk_f64<4, 16, true>},
+    {KIND_F64, 1, 19, true, (const void*)qbx_walk_f64<1, 19, true>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -47,11 +56,11 @@ constexpr uint32_t MAX_CHUNKS = 1u << 22;
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// largest available M <= min(want_M, n) for this kind, or -1
-static inline int pick_variant(int kind, int want_M, int n) {
+// largest available M <= min(want_M, n) for this kind (and record flag), or -1
+static inline int pick_variant(int kind, int want_M, int n, bool rec = false) {
     int best = -1;
     for (int v = 0; v < kNumVariants; ++v) {
-        if (kVariants[v].kind != kind) continue;
+        if (kVariants[v].kind != kind || kVariants[v].rec != rec) continue;
         if (kVariants[v].M <= want_M && kVariants[v].M <= n) {
             if (best < 0 || kVariants[v].M > kVariants[best].M) best = v;
         }
@@ -107,17 +116,21 @@ static inline int32_t plan_flags(const Plan& p, int32_t flags) {
     return flags;
 }
 
-// Tiling of one problem; returns QBX_OK or an error (message set).
+// Tiling of one problem; returns QBX_OK or an error (message set).  cap_L_at_B: lanes no
+// wider than the tie width B (every lane then lies inside one reference prefix), for
+// per-prefix records; needs B >= 1.
 static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mode, int index, Plan& p,
-                               const WalkTable* table = nullptr) {
+                               const WalkTable* table = nullptr, bool cap_L_at_B = false) {
     if (n < 1 || n > MAX_N) return set_err(QBX_EINVAL, "problem %d: n=%d outside [1, %d]", index, n, MAX_N);
     p.n = n;
     p.kind = kind;
     p.mode = mode;
     p.B = (o.tie_B < 0) ? (n < 20 ? n : 20) : o.tie_B;
     if (p.B > n) return set_err(QBX_EINVAL, "problem %d: tie_B=%d exceeds n=%d", index, p.B, n);
-    const int wantM = (o.inner_bits < 0) ? 7 : o.inner_bits;
-    p.variant = pick_variant(kind, wantM, n);
+    if (cap_L_at_B && p.B < 1) return set_err(QBX_EINVAL, "per-prefix records need tie_B >= 1");
+    int wantM = (o.inner_bits < 0) ? 7 : o.inner_bits;
+    if (cap_L_at_B && wantM > p.B) wantM = p.B;
+    p.variant = pick_variant(kind, wantM, n, cap_L_at_B);
     if (p.variant < 0) return set_err(QBX_EINVAL, "problem %d: no kernel for inner_bits=%d", index, wantM);
     p.M = kVariants[p.variant].M;
     const int OMAX = kVariants[p.variant].OMAX;
@@ -131,6 +144,7 @@ static inline int plan_problem(int kind, int n, const qbx_opts_t& o, PlanMode mo
         if (t >= p.M && t <= p.M + OMAX && t <= n && n - t <= MAX_LANE_BITS) L = t;
     }
     if (L < p.M) L = p.M;
+    if (cap_L_at_B && L > p.B) L = p.B;
     if (L > p.M + OMAX || L > n)
         return set_err(QBX_EINVAL, "problem %d: walk_bits=%d outside [%d, %d]", index, L, p.M,
                        (n < p.M + OMAX ? n : p.M + OMAX));
@@ -173,6 +187,7 @@ struct qbx_ctx {
     char* hbuf = nullptr;   // pinned staging
     size_t hcap = 0;
     int blocks_per_cu[qbx::kNumVariants];
+    int wall_khz = 0;       // rate of the device's wall clock (wall_clock64)
     qbx::WalkTable walk;    // tuned per-n walk bits (all -1 until qbx_set_walk_table)
 };
 

@@ -170,6 +170,10 @@ _SIGNATURES = {
     "qbx_device_info": (ctypes.c_int, [ctypes.c_void_p, _P(DevInfo)]),
     "qbx_solve_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts), _P(Result)]),
     "qbx_solve_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_solve_minima_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts),
+                                            _P(ctypes.c_int64), _P(ctypes.c_double), _P(Result)]),
+    "qbx_solve_minima_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts),
+                                            _P(ctypes.c_double), _P(ctypes.c_double), _P(Result)]),
     "qbx_solve_part_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts), ctypes.c_int,
                                           ctypes.c_int, _P(Result)]),
     "qbx_solve_part_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts), ctypes.c_int,
@@ -283,6 +287,19 @@ def _check(lib, rc: int):
 FLOAT_FILTERS = {None: 0, "int32": 1, "packed": 2}   # QBX_FLAG_FLOAT_INT32_FILTER / _PACKED_FILTER
 
 
+MAX_PREFIX_BITS = 28     # QBX_MAX_PREFIX_BITS: per-prefix records of at most 2^28 prefixes
+
+
+def _minima_buffers(n: int, kind: str, kw):
+    """Output arrays of a per-prefix solve (2^(n - B) entries) and its options."""
+    B = min(n, 20) if kw.get("tie_B") is None else int(kw["tie_B"])
+    if not (1 <= B <= n) or n - B > MAX_PREFIX_BITS:
+        raise ValueError(f"per-prefix records need 1 <= B <= N and N - B <= {MAX_PREFIX_BITS} (N={n}, B={B})")
+    count = 1 << (n - B)
+    minima = np.empty(count, dtype=np.int64 if kind == "i32" else np.float64)
+    return minima, np.empty(count, dtype=np.float64), _opts(**kw)
+
+
 def _opts(tie_B=None, walk_bits=None, inner_bits=None, float_filter=None) -> Opts:
     """float_filter: None (by problem size), "packed" (wherever it can run) or "int32"."""
     if float_filter not in FLOAT_FILTERS:
@@ -376,6 +393,21 @@ class Context:
                                         ctypes.byref(o), ctypes.byref(out))
         _check(self.lib, rc)
         return out
+
+    def solve_minima(self, Q: np.ndarray, kind: str = "i32", **kw):
+        """(Result, minima, wave_ms): the solve plus, per reference prefix in Gray visiting
+        order, its least energy (int64 / float64) and the device time spent on it (ms)."""
+        dtype, fn, ptr = ((np.int32, self.lib.qbx_solve_minima_i32, ctypes.c_int32) if kind == "i32"
+                          else (np.float64, self.lib.qbx_solve_minima_f64, ctypes.c_double))
+        Q = np.ascontiguousarray(Q, dtype=dtype)
+        minima, wave_ms, o = _minima_buffers(Q.shape[0], kind, kw)
+        out = Result()
+        with self.lock:
+            rc = fn(self._h, Q.ctypes.data_as(_P(ptr)), Q.shape[0], ctypes.byref(o),
+                    minima.ctypes.data_as(_P(ctypes.c_int64 if kind == "i32" else ctypes.c_double)),
+                    wave_ms.ctypes.data_as(_P(ctypes.c_double)), ctypes.byref(out))
+        _check(self.lib, rc)
+        return out, minima, wave_ms
 
     def solve_part(self, Q: np.ndarray, part: int, parts: int, kind: str = "i32", **kw) -> Result:
         """(E, rank) minimum over part `part` of `parts` equal ranges of the solve's chunks
@@ -580,6 +612,10 @@ def get_context(device: int = 0) -> Context:
 _CPU_SIGNATURES = {
     "qbx_cpu_solve_i32": (ctypes.c_int, [_P(ctypes.c_int32), ctypes.c_int, _P(Opts), ctypes.c_int, _P(Result)]),
     "qbx_cpu_solve_f64": (ctypes.c_int, [_P(ctypes.c_double), ctypes.c_int, _P(Opts), ctypes.c_int, _P(Result)]),
+    "qbx_cpu_solve_minima_i32": (ctypes.c_int, [_P(ctypes.c_int32), ctypes.c_int, _P(Opts), ctypes.c_int,
+                                                _P(ctypes.c_int64), _P(ctypes.c_double), _P(Result)]),
+    "qbx_cpu_solve_minima_f64": (ctypes.c_int, [_P(ctypes.c_double), ctypes.c_int, _P(Opts), ctypes.c_int,
+                                                _P(ctypes.c_double), _P(ctypes.c_double), _P(Result)]),
     "qbx_cpu_last_error": (ctypes.c_char_p, []),
 }
 CPU_EXPORTED_SYMBOLS = tuple(_CPU_SIGNATURES)
@@ -643,6 +679,20 @@ class CpuBackend:
 
     def solve_f64(self, Q: np.ndarray, **kw) -> Result:
         return self._solve(self.lib.qbx_cpu_solve_f64, Q, np.float64, ctypes.c_double, kw)
+
+    def solve_minima(self, Q: np.ndarray, kind: str = "i32", **kw):
+        """As Context.solve_minima, on the host (wave_ms: thread time per prefix)."""
+        dtype, fn, ptr = ((np.int32, self.lib.qbx_cpu_solve_minima_i32, ctypes.c_int32) if kind == "i32"
+                          else (np.float64, self.lib.qbx_cpu_solve_minima_f64, ctypes.c_double))
+        Q = np.ascontiguousarray(Q, dtype=dtype)
+        minima, wave_ms, o = _minima_buffers(Q.shape[0], kind, kw)
+        out = Result()
+        with self.lock:
+            rc = fn(Q.ctypes.data_as(_P(ptr)), Q.shape[0], ctypes.byref(o), self.threads,
+                    minima.ctypes.data_as(_P(ctypes.c_int64 if kind == "i32" else ctypes.c_double)),
+                    wave_ms.ctypes.data_as(_P(ctypes.c_double)), ctypes.byref(out))
+        _check_cpu(self.lib, rc)
+        return out, minima, wave_ms
 
     def batch(self, Qs, kind: str = "i32", packed=None, **kw):
         """One solve after another (each solve uses every thread); a problem outside its

@@ -129,6 +129,23 @@ def solve_matrix(Q, device=0, tie_B=None, walk_bits=None, inner_bits=None, float
     return _finish(p, r)
 
 
+def solve_matrix_minima(Q, device=0, tie_B=None, walk_bits=None, inner_bits=None, float_filter=None):
+    """solve_matrix's answer and, for the reference's per-prefix metrics (qbf.py:342-351,
+    362-365, 409-410), per reference prefix (top N - B bits) in Gray visiting order i
+    (prefix i ^ (i >> 1)): its least energy (int for integer problems' values) and the
+    device time spent on it in ms.  Returns (RawResult, minima, wave_ms); needs B >= 1
+    and N - B <= 28.  Lanes are capped at B bits (walk_bits / inner_bits too)."""
+    p = Q if isinstance(Q, Prepared) else prepare(Q)
+    if tie_B is not None and not (0 <= int(tie_B) <= p.n):
+        raise ValueError(f"B={tie_B} outside [0, N={p.n}]")
+    ctx = _native.get_backend(device)
+    r, minima, wave_ms = ctx.solve_minima(p.matrix, kind=p.kind, tie_B=tie_B, walk_bits=walk_bits,
+                                          inner_bits=inner_bits, float_filter=float_filter)
+    if p.kind == "f64" and p.integer:
+        minima = np.rint(minima).astype(np.int64)
+    return _finish(p, r), minima, wave_ms
+
+
 def solve_part(Q, part: int, parts: int, device=0, tie_B=None, walk_bits=None,
                inner_bits=None, float_filter=None) -> RawResult:
     """The (E, rank) minimum over one of `parts` equal shares of the solve's state space (one

@@ -128,6 +128,35 @@ class Parallel(SolverABC):
             raise ValueError(f"B={B} outside [0, N={N}]")
         return B
 
+    def _metrics(self, tasks, n_sub: int, kwargs) -> dict:
+        """The reference's merged metrics (parallel.py:245-267): every task's energy_arr
+        (plus its prefix_E) and time_arr, concatenated in task order.  A task's metrics are
+        those its own base solve would record: per prefix of the reduced problem under the
+        sub-solve's own A/B split (the kwargs applied to N - k variables), from one walk of
+        the reduced matrix (on the first device)."""
+        use = QBF.metrics_wanted(kwargs)
+        if not use or n_sub == 0:
+            return {}
+        kw = {k: v for k, v in kwargs.items() if k in ("A", "B", "use_mem_GB")}
+        kw.setdefault("size_of", 8)
+        _, B_sub = define_chunk_size(n_sub, **kw)
+        if self._base_solver is QB_F1:
+            B_sub = min(QB_F1.SUFFIX_BITS, n_sub)
+        if not (1 <= B_sub <= n_sub):
+            raise ValueError(f"use_metrics needs 1 <= B <= {n_sub} for the reduced problems (B={B_sub})")
+        dev = _native.parse_device(self._devices[0])
+        energy, times = [], []
+        for R, pe, _, _, R0 in tasks:
+            _, minima, wave_ms = engine.solve_matrix_minima(R0, device=dev, tie_B=B_sub)
+            energy.append(np.asarray(minima, dtype=np.float64) + float(pe))
+            times.append(wave_ms)
+        out = {}
+        if "time_arr" in use:
+            out["time_arr"] = np.concatenate(times)
+        if "energy_arr" in use:
+            out["energy_arr"] = np.concatenate(energy)
+        return out
+
     def solve(self, problem, timeout=None, **kwargs) -> Solution:
         SolverABC.check_obsolete_args(**kwargs)
         kw = dict(solver_type=self._solver_class_name, splits_cnt=self._splits_cnt,
@@ -149,12 +178,10 @@ class Parallel(SolverABC):
         tasks = []
         for task_id in range(self._splits_cnt):
             prefix = np.array([(task_id >> (k - 1 - j)) & 1 for j in range(k)], dtype=np.int64)
-            R, pe = reduced_matrix(Q, prefix)
+            R0, pe = reduced_matrix(Q, prefix)
             flip = flip_odd and bin(task_id).count("1") % 2 == 1
-            offset = 0
-            if flip:
-                R, offset = complement_first(R)
-            tasks.append((R, pe, flip, offset))
+            R, offset = complement_first(R0) if flip else (R0, 0)
+            tasks.append((R, pe, flip, offset, R0))
         ndev = len(self._devices)
         shares = [list(range(d, self._splits_cnt, ndev)) for d in range(ndev)]
         results: dict = {}
@@ -201,7 +228,7 @@ class Parallel(SolverABC):
         integer = np.issubdtype(Q.dtype, np.integer)
         for task_id in range(self._splits_cnt):
             _, r, _ = results[task_id]
-            R, pe, flip, offset = tasks[task_id]
+            R, pe, flip, offset, _ = tasks[task_id]
             if r is None:
                 e_sub, s_sub = 0, 0
             else:
@@ -221,7 +248,7 @@ class Parallel(SolverABC):
         merged.energies = np.array([np.float64(total)])
         merged.best_X = full
         merged.state_vectors = StateVectors(wide_vector_matrix=full.reshape(1, -1))
-        merged.metrics = [{}]
+        merged.metrics = [self._metrics(tasks, n_sub, kwargs)]
         worker_times: dict = {}
         for t, (w, _, ms) in results.items():
             worker_times[w] = worker_times.get(w, 0.0) + ms

@@ -20,8 +20,10 @@ kwargs (qbf.py:52-59, 429-441):
   correction_period  accepted for compatibility and not needed: the float path filters
                      in exact int32 arithmetic (no drift to correct) and computes every
                      energy it compares in FP64 from scratch.
-  use_metrics        'time_arr' / 'energy_arr' / 'all' (qbf.py:342-351); see
-                     pack_results for what each holds.
+  use_metrics        'time_arr' / 'energy_arr' / 'all' (qbf.py:342-351): per reference
+                     prefix, in Gray visiting order, its least energy (energy_arr) and the
+                     device (or host thread) time the walk spent on it (time_arr), both
+                     recorded by the same walk that finds the answer.
   walk_bits, inner_bits  tuning overrides (bits walked per lane, unrolled block bits).
 Obsolete kwargs use_gpu / gpu_id / devices raise ValueError as in the reference.
 """
@@ -71,10 +73,26 @@ class QBF(SolverABC):
 
     def solve(self, problem: ProblemABC, **kwargs) -> Solution:
         self.prepare(problem, **kwargs)
-        res = engine.solve_matrix(self._prepared, device=self.device_id, tie_B=self.B,
-                                  walk_bits=kwargs.get("walk_bits"), inner_bits=kwargs.get("inner_bits"))
+        kw = dict(device=self.device_id, tie_B=self.B, walk_bits=kwargs.get("walk_bits"),
+                  inner_bits=kwargs.get("inner_bits"))
+        self.prefix_metrics = None
+        if self.metrics_wanted(kwargs) and self.B >= 1:
+            # one walk: the answer and the per-prefix records together
+            res, minima, wave_ms = engine.solve_matrix_minima(self._prepared, **kw)
+            self.prefix_metrics = (minima, wave_ms)
+        else:
+            res = engine.solve_matrix(self._prepared, **kw)
         self.timer.pick_time("Kernel run time")
         return self.pack_results_to_solution_instance(problem, res, **kwargs)
+
+    @staticmethod
+    def metrics_wanted(kwargs) -> set:
+        use_metrics = kwargs.get("use_metrics", set())
+        if use_metrics == "all":
+            return set(ALL_METRICS)
+        if isinstance(use_metrics, str):
+            return {use_metrics}
+        return set(use_metrics)
 
     def prepare(self, problem: ProblemABC, **kwargs):
         self.check_obsolete_args(**kwargs)
@@ -105,16 +123,16 @@ class QBF(SolverABC):
         sol.state_vectors = StateVectors(wide_vector_matrix=x.reshape(1, n))
         E = np.float64(res.E)
         sol.energies = np.array([E])
-        use_metrics = kwargs.get("use_metrics", set())
-        if use_metrics == "all":
-            use_metrics = ALL_METRICS
+        use_metrics = self.metrics_wanted(kwargs)
         metric_dict = {}
-        if "time_arr" in use_metrics:
-            # the reference times each of its 2^A host-loop steps; a single kernel walks
-            # all prefixes at once, so its one device interval is reported
-            metric_dict["time_arr"] = np.array([res.info["kernel_ms"]])
-        if "energy_arr" in use_metrics:
-            metric_dict["energy_arr"] = self.prefix_minima(problem)
+        if use_metrics:
+            minima, wave_ms = self.prefix_metrics if self.prefix_metrics is not None else self.state_metrics()
+            if "time_arr" in use_metrics:
+                # the reference times each of its 2^A host-loop steps (qbf.py:362-365); here
+                # all prefixes are walked at once, and each gets the time spent on its states
+                metric_dict["time_arr"] = np.asarray(wave_ms, dtype=np.float64)
+            if "energy_arr" in use_metrics:
+                metric_dict["energy_arr"] = np.asarray(minima, dtype=np.float64)
         sol.metrics = [metric_dict]
         sol.solver_class = self.solver_class
         sol.best_E = E
@@ -129,30 +147,21 @@ class QBF(SolverABC):
         sol.run_info = dict(res.info, tie_B=res.tie_B, state=res.state, rank=res.rank)
         return sol
 
-    def prefix_minima(self, problem) -> np.ndarray:
-        """energy_arr: for each of the 2^A reference prefixes, in Gray visiting order, the
-        minimum energy over its 2^B suffixes (qbf.py:409-410).  Computed exactly on the
-        GPU as 2^A reduced problems of size B in one batch."""
-        from ..problems import ReducedProblem, Just_Q
-        A, B = self.A, self.B
-        if A > 16:
-            raise ValueError(f"energy_arr needs 2^A problems; A={A} is too large (max 16)")
-        base = Just_Q(self.Q)
-        out = np.empty(2 ** A)
-        subs, offs = [], []
-        for i in range(2 ** A):
-            g = i ^ (i >> 1)
-            prefix = np.array([(g >> (A - 1 - j)) & 1 for j in range(A)], dtype=np.int64)
-            rp = ReducedProblem(base, prefix)
-            q = rp.build_Q_matrix()
-            subs.append(q)
-            offs.append(rp.prefix_E)
-        if B == 0:
-            return np.array(offs, dtype=np.float64)
-        res = engine.solve_matrices(subs, device=self.device_id)
-        for i, (r, pe) in enumerate(zip(res, offs)):
-            out[i] = float(r.E) + float(pe)
-        return out
+    STATE_METRICS_MAX_N = 20
+
+    def state_metrics(self):
+        """Metrics for B = 0, where every reference prefix is one state: the energies of
+        the 2^N states in Gray visiting order (N <= 20), and no per-prefix time."""
+        N = self.N
+        if N > self.STATE_METRICS_MAX_N:
+            raise ValueError(f"use_metrics with B=0 lists all 2^N energies: N={N} exceeds "
+                             f"{self.STATE_METRICS_MAX_N}")
+        i = np.arange(2 ** N, dtype=np.int64)
+        g = i ^ (i >> 1)
+        X = ((g[:, None] >> np.arange(N - 1, -1, -1, dtype=np.int64)) & 1).astype(np.float64)
+        Qf = np.asarray(self.Q, dtype=np.float64)
+        E = ((X @ Qf) * X).sum(axis=1)
+        return E, np.zeros(2 ** N)
 
     def clear_mem(self):
         self.Q = None

@@ -123,6 +123,25 @@ int qbx_solve_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opt
 int qbx_solve_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts,
                   qbx_result_t *out);
 
+/* A solve with the reference's per-prefix metrics beside its answer (`out` is exactly
+ * qbx_solve_*'s).  The reference walks the 2^A prefixes P (the top A = n - B bits of the
+ * state, B = tie_B) in Gray order, step i visiting P = i ^ (i >> 1), and records each step's
+ * best energy (energy_arr, qbf.py:342-351, 409-410) and duration (time_arr, qbf.py:362-365).
+ * Here, for i = 0 .. 2^A - 1:
+ *   minima[i]  = the least energy over the 2^B states of prefix i ^ (i >> 1)
+ *                (int64 for qbx_solve_minima_i32, double -- FP64-exact -- for _f64);
+ *   wave_ms[i] = (nullable) device time the walk spent on that prefix: the wall time of the
+ *                waves that walked its states, each wave's share prorated by its lanes, in ms
+ *                (the waves run concurrently, so the sum exceeds the solve's duration).
+ * The walk then cuts lanes no wider than B bits (walk_bits is capped at B, inner_bits at B);
+ * float problems walk without the shared bound across lanes (every lane finds its own
+ * minimum), so they take longer than qbx_solve_f64.  Needs 1 <= B and A <= 28. */
+#define QBX_MAX_PREFIX_BITS 28
+int qbx_solve_minima_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opts, int64_t *minima,
+                         double *wave_ms, qbx_result_t *out);
+int qbx_solve_minima_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts, double *minima,
+                         double *wave_ms, qbx_result_t *out);
+
 /* One part of a solve: the states in part `part` of `parts` equal, contiguous ranges of the
  * solve's chunks (chunks are runs of tiles, tiles runs of the state's leading bits; needs
  * parts <= chunks).  out holds the (E, rank) minimum over that part (out.states, out.tiles:

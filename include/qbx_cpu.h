@@ -32,6 +32,14 @@ int qbx_cpu_solve_i32(const int32_t *Q, int n, const qbx_opts_t *opts, int threa
  * reference counterpart, for float Q (there with float32 accumulation and periodic FP64
  * correction, qbf.py:214-217, 232-244). */
 int qbx_cpu_solve_f64(const double *Q, int n, const qbx_opts_t *opts, int threads, qbx_result_t *out);
+/* The solve with the reference's per-prefix metrics beside its answer, as
+ * qbx_solve_minima_* (qbx.h; energy_arr and time_arr, qbf.py:342-351, 362-365, 409-410):
+ * minima[i] = least energy over the states of reference prefix i ^ (i >> 1), wave_ms[i]
+ * (nullable) = thread time spent on that prefix, ms.  Needs 1 <= B, n - B <= 28. */
+int qbx_cpu_solve_minima_i32(const int32_t *Q, int n, const qbx_opts_t *opts, int threads, int64_t *minima,
+                             double *wave_ms, qbx_result_t *out);
+int qbx_cpu_solve_minima_f64(const double *Q, int n, const qbx_opts_t *opts, int threads, double *minima,
+                             double *wave_ms, qbx_result_t *out);
 const char *qbx_cpu_last_error(void);
 
 #ifdef __cplusplus

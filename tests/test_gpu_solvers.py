@@ -68,7 +68,7 @@ def test_qbf_energy_arr_metric():
         g = i ^ (i >> 1)
         states = [(g << 8) | t for t in range(256)]
         assert arr[i] == min(oracle.energy(Q, s) for s in states)
-    assert sol.metrics[0]["time_arr"].shape == (1,)
+    assert sol.metrics[0]["time_arr"].shape == (16,)
 
 
 def test_qbf1_contract():
