@@ -34,15 +34,14 @@
 using namespace qbx;
 
 namespace qbx {
-static thread_local std::string g_err;
+// the calling thread's last message (a fixed buffer: setting it never allocates)
+static thread_local char g_err[512];
 
 int set_err(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 // Device arena of one batch: [instances | ranges | Q data] (staged from the host in one
@@ -205,6 +204,24 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
     return QBX_OK;
 }
 
+// Descriptor of one problem whose sections sit in the device arena at dv (red and result:
+// its reduction state and result record there).
+void fill_inst(const Plan& p, int kind, int32_t flags, uint32_t expect, char* dv, RedDev* red, ResultDev* result,
+               InstDev& I) {
+    memset(&I, 0, sizeof I);
+    I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = kind; I.status = 0; I.tiles = p.tiles;
+    I.tpc = p.tpc; I.chunks = p.chunks; I.flags = plan_flags(p, flags);
+    I.expect = expect;
+    I.Q = dv + p.off_Q;
+    I.W = dv + p.off_W;
+    I.d = dv + p.off_d;
+    I.delta = dv + p.off_delta;
+    I.red = red;
+    I.cand = (uint32_t*)(dv + p.off_cand);
+    I.slots = (SlotDev*)(dv + p.off_slots);
+    I.result = result;
+}
+
 // Instance descriptors and tile ranges (device pointers into the arena at dv) into the
 // host staging buffer h.
 void stage_descriptors(const Layout& lay, char* h, char* dv) {
@@ -212,18 +229,8 @@ void stage_descriptors(const Layout& lay, char* h, char* dv) {
     for (int i = 0; i < lay.count; ++i) {
         const Plan& p = lay.plans[i];
         InstDev& I = hinst[i];
-        memset(&I, 0, sizeof I);
-        I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = lay.kind; I.status = 0; I.tiles = p.tiles;
-        I.tpc = p.tpc; I.chunks = p.chunks; I.flags = plan_flags(p, lay.flags);
-        I.expect = lay.part_count ? lay.part_count : p.chunks;
-        I.Q = dv + p.off_Q;
-        I.W = dv + p.off_W;
-        I.d = dv + p.off_d;
-        I.delta = dv + p.off_delta;
-        I.red = (RedDev*)(dv + lay.off_red) + i;
-        I.cand = (uint32_t*)(dv + p.off_cand);
-        I.slots = (SlotDev*)(dv + p.off_slots);
-        I.result = (ResultDev*)(dv + lay.off_res) + i;
+        fill_inst(p, lay.kind, lay.flags, lay.part_count ? lay.part_count : p.chunks, dv,
+                  (RedDev*)(dv + lay.off_red) + i, (ResultDev*)(dv + lay.off_res) + i, I);
         if (lay.minima) {
             I.minima = (uint64_t*)(dv + lay.off_minima);
             I.ticks = (uint64_t*)(dv + lay.off_ticks);
@@ -312,29 +319,33 @@ int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const c
     return QBX_OK;
 }
 
+// The caller's result record of one problem from its device result.
+void fill_result(const Plan& p, int kind, const ResultDev& h, float ms, qbx_result_t& R) {
+    memset(&R, 0, sizeof R);
+    R.n = p.n; R.tie_B = p.B; R.walk_bits = p.L; R.inner_bits = p.M; R.lane_bits = p.H;
+    R.states = (p.n >= 64) ? ~0ull : (1ull << p.n);
+    R.tiles = p.tiles;
+    R.kernel_ms = ms;
+    R.status = h.status;
+    R.path = h.path;
+    R.rank = h.rank;
+    R.state = h.state;
+    if (kind == KIND_I32) {
+        R.E_int = (int64_t)h.Ebits;
+        R.E = (double)R.E_int;
+    } else {
+        double e;
+        memcpy(&e, &h.Ebits, sizeof e);
+        R.E = e;
+        R.E_int = (int64_t)e;
+    }
+}
+
 int collect(const Layout& lay, const ResultDev* hres, float ms, qbx_result_t* out) {
     int first_bad = QBX_OK;
     for (int i = 0; i < lay.count; ++i) {
-        const Plan& p = lay.plans[i];
         qbx_result_t& R = out[i];
-        memset(&R, 0, sizeof R);
-        R.n = p.n; R.tie_B = p.B; R.walk_bits = p.L; R.inner_bits = p.M; R.lane_bits = p.H;
-        R.states = (p.n >= 64) ? ~0ull : (1ull << p.n);
-        R.tiles = p.tiles;
-        R.kernel_ms = ms;
-        R.status = hres[i].status;
-        R.path = hres[i].path;
-        R.rank = hres[i].rank;
-        R.state = hres[i].state;
-        if (lay.kind == KIND_I32) {
-            R.E_int = (int64_t)hres[i].Ebits;
-            R.E = (double)R.E_int;
-        } else {
-            double e;
-            memcpy(&e, &hres[i].Ebits, sizeof e);
-            R.E = e;
-            R.E_int = (int64_t)e;
-        }
+        fill_result(lay.plans[i], lay.kind, hres[i], ms, R);
         if (R.status != 0 && first_bad == QBX_OK) {
             first_bad = R.status;
             set_err(R.status, "problem %d: energy bound outside the %s path", i,
@@ -534,11 +545,56 @@ int plan_run(qbx_batch_plan* P, int kind, const T* Qpacked, const int64_t* offse
     return collect(P->lay, (const ResultDev*)P->h_res, ms, out);
 }
 
+// Sections of one single int32 solve in one buffer, in make_layout's order for count = 1:
+// [InstDev | TileRange | Q] (staged), tables, reduction state, candidate ids, result
+// slots, the result, the tickets.
+struct SingleLayout {
+    size_t off_inst, off_ranges, stage_bytes, off_red, off_res, off_ticket, total;
+};
+
+void single_layout(Plan& p, int kind, SingleLayout& S) {
+    size_t off = 0;
+    S.off_inst = off;
+    off = align256(off + sizeof(InstDev));
+    S.off_ranges = off;
+    off = align256(off + sizeof(TileRange));
+    p.off_Q = off;
+    off = align256(off + (kind == KIND_I32 ? 4 : 8) * (size_t)p.n * p.n);
+    S.stage_bytes = off;
+    p.off_W = off;
+    off = align256(off + table_W_bytes(kind, p.n));
+    p.off_d = off;
+    off = align256(off + table_d_bytes(kind));
+    p.off_delta = off;
+    off = align256(off + table_delta_bytes(kind, p.M));
+    S.off_red = off;
+    off = align256(off + sizeof(RedDev));
+    p.off_cand = off;
+    off = align256(off + 4 * (size_t)p.chunks);
+    p.off_slots = off;
+    off = align256(off + sizeof(SlotDev) * (size_t)p.chunks);
+    S.off_res = off;
+    off = align256(off + sizeof(ResultDev));
+    S.off_ticket = off;
+    off = align256(off + sizeof(unsigned long long) * MAX_GROUPS);
+    S.total = off;
+}
+
+// Host staging of qbx_solve_i32_noalloc, on the caller's stack: descriptor, tile range and
+// the largest Q (56 x 56 int32).
+constexpr size_t NOALLOC_STAGE = 256 + 256 + 12800;
+static_assert(sizeof(InstDev) <= 256 && sizeof(TileRange) <= 256, "staging sections");
+
 }  // namespace
+
+// Device scratch of qbx_solve_i32_noalloc: a file-scope array each GPU holds (from module
+// load to the end of the process), shared by every call on that GPU.  Host code never uses
+// its name as an address: qbx_open takes its per-device address with hipGetSymbolAddress.
+static __device__ __align__(256) char scratch[QBX_NOALLOC_SCRATCH_BYTES];
 
 extern "C" {
 
-const char* qbx_last_error(void) { return g_err.c_str(); }
+const char* qbx_last_error(void) { return g_err; }
 
 int qbx_open(int device, qbx_ctx_t** out) {
     if (!out) return set_err(QBX_EINVAL, "null output pointer");
@@ -555,6 +611,8 @@ int qbx_open(int device, qbx_ctx_t** out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device);
+    // this device's copy of the no-alloc scratch array (device `device` is current here)
+    if (e == hipSuccess) e = hipGetSymbolAddress((void**)&c->scratch, HIP_SYMBOL(scratch));
     for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
         int b = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kVariants[v].fn, 64, 0);
@@ -605,6 +663,64 @@ int qbx_solve_f64(qbx_ctx_t* c, const double* Q, int n, const qbx_opts_t* opts, 
     const int64_t off = 0;
     const int32_t nn = n;
     return run_batch<double>(c, KIND_F64, 1, Q, &off, &nn, opts, out, true);
+}
+
+int qbx_solve_i32_noalloc(qbx_ctx_t* c, const int32_t* Q, int n, const qbx_opts_t* opts, qbx_result_t* out) {
+    if (!c || !Q || !out) return set_err(QBX_EINVAL, "null argument");
+    if (!c->scratch) return set_err(QBX_EDEVICE, "no scratch address for device %d", c->device);
+    qbx_opts_t o = {-1, -1, -1, 0};
+    if (opts) o = *opts;
+    if (o.flags & ~(QBX_FLAG_FLOAT_INT32_FILTER | QBX_FLAG_FLOAT_PACKED_FILTER))
+        return set_err(QBX_EINVAL, "unknown opts.flags bits 0x%x", o.flags);
+    Plan p;
+    int rc = plan_problem(KIND_I32, n, o, PLAN_SINGLE, 0, p, &c->walk);
+    if (rc) return rc;
+    SingleLayout S;
+    single_layout(p, KIND_I32, S);
+    if (S.total > (size_t)QBX_NOALLOC_SCRATCH_BYTES)
+        return set_err(QBX_EINVAL,
+                       "n=%d with walk_bits=%d needs %zu bytes of device scratch; qbx_solve_i32_noalloc has %d "
+                       "(use qbx_solve_i32, or more walk bits)", n, p.L, S.total, QBX_NOALLOC_SCRATCH_BYTES);
+    alignas(256) char h[NOALLOC_STAGE];
+    if (S.stage_bytes > sizeof h) return set_err(QBX_EINVAL, "staging of n=%d exceeds %zu bytes", n, sizeof h);
+    char* dv = c->scratch;    // this device's scratch array
+    fill_inst(p, KIND_I32, o.flags, p.chunks, dv, (RedDev*)(dv + S.off_red), (ResultDev*)(dv + S.off_res),
+              *(InstDev*)(h + S.off_inst));
+    TileRange& R = *(TileRange*)(h + S.off_ranges);
+    R.inst = 0; R.c_begin = 0; R.c_count = p.chunks; R.pad = 0; R.prefix = 0;
+    memcpy(h + p.off_Q, Q, sizeof(int32_t) * (size_t)n * n);
+    HIP_TRY(hipSetDevice(c->device));
+    // everything below is enqueued on the context stream, and the stream is drained before
+    // returning on every path (the staging buffer lives on this stack frame)
+    hipError_t e = hipMemcpyAsync(dv, h, S.stage_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) {
+        // qbx_prep resets the reduction state and the tickets: nothing starts from the last
+        // call's contents
+        hipLaunchKernelGGL(qbx_prep, dim3(1), dim3(256), 0, c->stream, (InstDev*)(dv + S.off_inst),
+                           (const uint32_t*)nullptr, (unsigned long long*)(dv + S.off_ticket), (const InstDev*)nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        const int v = p.variant;
+        const unsigned long long grid_full = (unsigned long long)c->blocks_per_cu[v] * c->prop.multiProcessorCount;
+        const unsigned long long grid = std::max(1ull, std::min(grid_full, (unsigned long long)p.chunks));
+        hipLaunchKernelGGL((walk_fn_t)kVariants[v].fn, dim3((unsigned)grid), dim3(64), 0, c->stream,
+                           (const InstDev*)(dv + S.off_inst), (const TileRange*)(dv + S.off_ranges), 1,
+                           (unsigned long long)p.chunks, (unsigned long long*)(dv + S.off_ticket), 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    ResultDev hres;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hres, dv + S.off_res, sizeof hres, hipMemcpyDeviceToHost, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_err(QBX_EDEVICE, "qbx_solve_i32_noalloc: %s", hipGetErrorString(e));
+    if (es != hipSuccess) return set_err(QBX_EDEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(es));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    fill_result(p, KIND_I32, hres, ms, *out);
+    if (out->status != 0) return set_err(out->status, "energy bound outside the int32 path");
+    return QBX_OK;
 }
 
 int qbx_solve_minima_i32(qbx_ctx_t* c, const int32_t* Q, int n, const qbx_opts_t* opts, int64_t* minima,

@@ -188,6 +188,7 @@ struct qbx_ctx {
     size_t hcap = 0;
     int blocks_per_cu[qbx::kNumVariants];
     int wall_khz = 0;       // rate of the device's wall clock (wall_clock64)
+    char* scratch = nullptr;  // this device's qbx_solve_i32_noalloc scratch (hipGetSymbolAddress)
     qbx::WalkTable walk;    // tuned per-n walk bits (all -1 until qbx_set_walk_table)
 };
 

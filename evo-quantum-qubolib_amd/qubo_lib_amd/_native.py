@@ -170,6 +170,8 @@ _SIGNATURES = {
     "qbx_device_info": (ctypes.c_int, [ctypes.c_void_p, _P(DevInfo)]),
     "qbx_solve_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts), _P(Result)]),
     "qbx_solve_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts), _P(Result)]),
+    "qbx_solve_i32_noalloc": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts),
+                                             _P(Result)]),
     "qbx_solve_minima_i32": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), ctypes.c_int, _P(Opts),
                                             _P(ctypes.c_int64), _P(ctypes.c_double), _P(Result)]),
     "qbx_solve_minima_f64": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), ctypes.c_int, _P(Opts),
@@ -391,6 +393,18 @@ class Context:
         with self.lock:
             rc = self.lib.qbx_solve_f64(self._h, Q.ctypes.data_as(_P(ctypes.c_double)), n,
                                         ctypes.byref(o), ctypes.byref(out))
+        _check(self.lib, rc)
+        return out
+
+    def solve_i32_noalloc(self, Q: np.ndarray, **kw) -> Result:
+        """qbx_solve_i32 in the library's fixed device scratch (no allocation; calls that
+        overlap on one GPU race)."""
+        Q = np.ascontiguousarray(Q, dtype=np.int32)
+        out = Result()
+        o = _opts(**kw)
+        with self.lock:
+            rc = self.lib.qbx_solve_i32_noalloc(self._h, Q.ctypes.data_as(_P(ctypes.c_int32)), Q.shape[0],
+                                                ctypes.byref(o), ctypes.byref(out))
         _check(self.lib, rc)
         return out
 

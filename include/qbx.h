@@ -123,6 +123,20 @@ int qbx_solve_i32(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opt
 int qbx_solve_f64(qbx_ctx_t *ctx, const double *Q, int n, const qbx_opts_t *opts,
                   qbx_result_t *out);
 
+/* qbx_solve_i32 for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * kernels, same answer and result fields as qbx_solve_i32; it calls no hipMalloc, hipFree,
+ * hipHostMalloc nor any other allocating routine: the solve's device memory (tables,
+ * reduction state, per-chunk slots) is a fixed array of QBX_NOALLOC_SCRATCH_BYTES that every
+ * GPU holds from module load to the end of the process, and Q is staged from the stack.
+ * A solve needing more scratch than that (n >= 44 at the default tiling: 2^20 and more
+ * per-chunk slots; more walk bits shrink them) is rejected with QBX_EINVAL.  All calls on
+ * one GPU share that GPU's array: calls that overlap on one GPU (from several threads or
+ * contexts) race.  Each call resets what must start at zero (reduction state, tickets). */
+#define QBX_NOALLOC_SCRATCH_BYTES (16 << 20)
+int qbx_solve_i32_noalloc(qbx_ctx_t *ctx, const int32_t *Q, int n, const qbx_opts_t *opts,
+                          qbx_result_t *out);
+
 /* A solve with the reference's per-prefix metrics beside its answer (`out` is exactly
  * qbx_solve_*'s).  The reference walks the 2^A prefixes P (the top A = n - B bits of the
  * state, B = tie_B) in Gray order, step i visiting P = i ^ (i >> 1), and records each step's
