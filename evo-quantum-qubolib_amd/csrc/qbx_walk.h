@@ -353,6 +353,26 @@ QBX_HD void trans_dyn_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ WJ, i
 
 constexpr int GROUP_BITS = 4;   // blocks are walked in unrolled groups of 16
 
+// Issue priority of the wave from its progress through the lane (c of n blocks): 3 in the
+// first quarter down to 0 in the last.  Two waves share a SIMD and the sequencer favours
+// the older one, so without this the younger wave of a pair runs behind and finishes its
+// walk alone, latency-bound; lowering the priority of the wave that is ahead lets the other
+// catch up, and the pair finishes together.  Used by the float filters, whose single solves
+// run about one tile per wave (measured on MI355X: n = 34 float 0.63 -> 0.60 ms); the
+// integer walks run without it (it cost the n = 40 rebased walk 3%).  Host build: nothing.
+QBX_HD void wave_priority_by_progress(uint32_t c, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t level = (uint32_t)(((uint64_t)c * 4u) / (n ? n : 1u));   // wave-uniform
+    if (level == 0) __builtin_amdgcn_s_setprio(3);
+    else if (level == 1) __builtin_amdgcn_s_setprio(2);
+    else if (level == 2) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+#else
+    (void)c;
+    (void)n;
+#endif
+}
+
 // E0 and inner fields of an arbitrary block, from scratch: hs = the state's bits above the
 // inner block (bit j of hs is state bit M + j).  Used once per lane, for its best block.
 template <int M>
@@ -797,6 +817,26 @@ QBX_HD void load_wrow_rp(WRowRP<M>& r, const int32_t* __restrict__ W, const uint
     constexpr int J = M + 1 + I;
     const uint32_t* WPJ = Wp + J * LDW;
     const int32_t* WJ = W + J * LDW;
+#pragma unroll
+    for (int k = 0; k < M; ++k) r.p[k] = WPJ[k];
+    r.w[0] = WJ[M];
+#pragma unroll
+    for (int q = 0; q < I; ++q) r.w[1 + q] = WJ[M + 1 + q];
+}
+
+// The same row read through the constant address space: the tables are written by an
+// earlier launch and only read by the walks, and the address is wave-uniform, so on the
+// device these become scalar loads into SGPRs (scalar cache, short latency, no VGPRs).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QBX_CONST_AS __attribute__((address_space(4)))
+#else
+#define QBX_CONST_AS
+#endif
+template <int M, int I>
+QBX_HD void load_wrow_rp_uniform(WRowRP<M>& r, const int32_t* __restrict__ W, const uint32_t* __restrict__ Wp) {
+    constexpr int J = M + 1 + I;
+    const QBX_CONST_AS uint32_t* WPJ = (const QBX_CONST_AS uint32_t*)(uintptr_t)(Wp + J * LDW);
+    const QBX_CONST_AS int32_t* WJ = (const QBX_CONST_AS int32_t*)(uintptr_t)(W + J * LDW);
 #pragma unroll
     for (int k = 0; k < M; ++k) r.p[k] = WPJ[k];
     r.w[0] = WJ[M];
@@ -1341,6 +1381,7 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
 
     const uint32_t nblk = 1u << O;
     for (uint32_t c0 = 0;;) {
+        wave_priority_by_progress(c0, nblk);
         group_q32<M, OMAX>(s, Wq, dlq, c0, nblk, q, b, std::make_integer_sequence<int, (1 << GROUP_BITS)>{});
         if (b.count > QCAP - (1 << GROUP_BITS)) {   // room for one more group of records
             q_compact<M>(b, q, x, false, bestE, bestR);
@@ -1511,9 +1552,10 @@ QBX_HD bool group_step_qrp(LaneRP<M, OMAX>& s, uint32_t* F, const int32_t* __res
         constexpr int i = J > 0 ? __builtin_ctz((unsigned)J) : 0;
         const bool up = (i + 1 < GROUP_BITS) ? ((((unsigned)J >> (i + 1)) & 1u) == 0u)
                                              : (((c0 >> GROUP_BITS) & 1u) == 0u);
-        // (no row prefetch: its registers would cost this kernel its second wave)
+        // the row through the scalar cache (no prefetch: its VGPRs would cost this kernel
+        // its second wave)
         WRowRP<M> row;
-        load_wrow_rp<M, i>(row, Wq, Wpc);
+        load_wrow_rp_uniform<M, i>(row, Wq, Wpc);
         trans_row_rp<M, OMAX, i>(s, F, row, up);
     }
     const uint32_t mp = block_min_pair<M>((uint32_t)PAIR_BIAS * 65537u, F, dl);
@@ -1605,6 +1647,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
     const int32_t* W3 = Wq + (M + GROUP_BITS) * LDW;    // row of group bit GROUP_BITS - 1
     const uint32_t nblk = 1u << (O - 1);
     for (uint32_t c0 = 0;;) {
+        wave_priority_by_progress(c0, nblk);
         int32_t fc[M];
         uint32_t g = 0;
 #pragma unroll
@@ -1664,6 +1707,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
         const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
         trans_dyn_rp<M, OMAX>(s, Wq + (M + 1 + i) * LDW, i, up);
     }
+    wave_priority_by_progress(nblk, nblk);
     q_compact<M, true>(b, q, x, true, bestE, bestR);
     LaneOutF64 out;
     out.E = bestE;
