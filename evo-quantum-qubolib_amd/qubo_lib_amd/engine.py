@@ -8,6 +8,10 @@ Arithmetic path per problem:
   * integer Q with bound < 2^53: the FP64 walk, exact on integers (qbx_solve_f64).
   * float Q whose entries are all integers (MaxCut's default float32 matrices): the
       exact int32 walk, energies returned as floats.
+  * float Q whose entries are all multiples of one power of two u = 2^-k (weights / 4,
+      quarter- or half-integer matrices), with an energy bound of at most 2^31 - 2 units
+      of u: the same, on Q / u (an exact integer problem), energies u * E_int -- exact
+      values, and the same order and ties as the float path gives them.
   * other float Q: the FP64 walk on the float64 values of Q (the int32 walk over tables
       quantized to 2^s units filters blocks with a proven error bound; every block it
       passes is re-walked in FP64).
@@ -39,6 +43,7 @@ class Prepared:
     integer: bool             # the problem's values are integers
     matrix: np.ndarray        # int32 (folded) or float64
     bound: float
+    unit: float = 1.0         # i32 kind: energies are unit * the integer walk's (a power of two)
 
 
 @dataclass
@@ -57,7 +62,9 @@ def default_tie_B(n: int) -> int:
     return min(n, 20)
 
 
-def prepare(Q) -> Prepared:
+def prepare(Q, lattice: bool = True) -> Prepared:
+    """Validate Q and choose its arithmetic path.  lattice=False keeps float matrices whose
+    entries are multiples of a power of two on the float path (same answer, slower)."""
     Q = np.asarray(Q)
     if Q.ndim != 2 or Q.shape[0] != Q.shape[1]:
         raise ValueError(f"Q must be a square matrix, got shape {Q.shape}")
@@ -99,16 +106,45 @@ def prepare(Q) -> Prepared:
                 p = None
             if p is not None and p.kind == "i32":
                 return Prepared(p.n, "i32", False, p.matrix, p.bound)
+        k = lattice_bits(Qf) if lattice else -1
+        if 0 < k <= 62:
+            # Q = u * Z with u = 2^-k and Z integral: the integer walks on Z, exactly
+            Z = np.ldexp(Qf, k)
+            if np.abs(Z).max(initial=0.0) < 2.0 ** 53:
+                try:
+                    p = prepare(Z.astype(np.int64))
+                except OverflowError:
+                    p = None
+                if p is not None and p.kind == "i32":
+                    return Prepared(p.n, "i32", False, p.matrix, p.bound, unit=float(np.ldexp(1.0, -k)))
         bound = float(np.abs(np.diag(Qf)).sum() + np.abs(np.triu(Qf, 1) + np.triu(Qf.T, 1)).sum())
         return Prepared(n, "f64", False, np.ascontiguousarray(Qf), bound)
     raise TypeError(f"unsupported Q dtype {Q.dtype}")
+
+
+def lattice_bits(Qf: np.ndarray) -> int:
+    """Smallest k >= 0 with Qf * 2^k all integers (every finite double is a multiple of a
+    power of two: its odd integer mantissa times 2^e), or -1 when it would exceed 1074."""
+    v = Qf[Qf != 0.0]
+    if v.size == 0:
+        return 0
+    m, e = np.frexp(np.abs(v))                  # |v| = m 2^e, 0.5 <= m < 1
+    M = np.ldexp(m, 53).astype(np.int64)        # the 53-bit integer mantissa, exact
+    tz = np.log2((M & -M).astype(np.float64)).astype(np.int64)   # its trailing zero bits
+    k = int((53 - e - tz).max())                # |v| = (M >> tz) 2^(e - 53 + tz)
+    return max(k, 0) if k <= 1074 else -1
+
+
+def energy_of(p: Prepared, E_int) -> float:
+    """Energy of an integer-walk result of a float problem (unit * E_int, exact)."""
+    return float(E_int) * p.unit
 
 
 def _finish(p: Prepared, r: _native.Result) -> RawResult:
     if p.integer:
         E = int(r.E_int) if p.kind == "i32" else int(round(r.E))
     elif p.kind == "i32":
-        E = float(r.E_int)
+        E = energy_of(p, r.E_int)
     else:
         E = float(r.E)
     info = dict(kernel_ms=r.kernel_ms, states=int(r.states), tiles=int(r.tiles), walk_bits=r.walk_bits,
@@ -143,6 +179,8 @@ def solve_matrix_minima(Q, device=0, tie_B=None, walk_bits=None, inner_bits=None
                                           inner_bits=inner_bits, float_filter=float_filter)
     if p.kind == "f64" and p.integer:
         minima = np.rint(minima).astype(np.int64)
+    elif p.kind == "i32" and not p.integer:
+        minima = minima.astype(np.float64) * p.unit
     return _finish(p, r), minima, wave_ms
 
 

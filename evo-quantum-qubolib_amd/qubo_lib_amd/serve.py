@@ -94,7 +94,7 @@ class Server:
                 if p.integer:
                     r["E"] = int(r["E_int"]) if p.kind == "i32" else int(round(r["E"]))
                 elif p.kind == "i32":
-                    r["E"] = float(r["E_int"])
+                    r["E"] = engine.energy_of(p, r["E_int"])
             out.append(r)
         return out
 

@@ -61,10 +61,14 @@ def test_float_exact_ties():
     for t in range(40):
         n = int(rng.integers(2, 15))
         Q = rng.integers(-4, 5, (n, n)) / 4.0 + 0.125 * (t % 2)
-        assert engine.prepare(Q).kind == "f64"
+        p = engine.prepare(Q, lattice=False)
+        assert p.kind == "f64"
         B = int(rng.integers(0, n + 1))
-        r = engine.solve_matrix(Q, device="cpu", tie_B=B)
+        r = engine.solve_matrix(p, device="cpu", tie_B=B)
         assert (r.E, r.state) == oracle.brute_f64(Q, B)
+        # the same matrix as the exact integer problem 8 Q: identical answer
+        z = engine.solve_matrix(Q, device="cpu", tie_B=B)
+        assert z.kind == "i32" and (z.E, z.state) == (r.E, r.state)
 
 
 def test_reference_cpu_configuration_n24():
@@ -163,3 +167,22 @@ def test_parallel_tie_key_with_explicit_widths():
         for k in range(0, min(n, 4) + 1):
             sol = Parallel(QBF, 2 ** k, ["cpu"]).solve(Just_Q(Q), **kw)
             assert bits(sol.best_X) == bits(whole.best_X), (n, kw, k)
+
+
+def test_power_of_two_lattice_floats_take_the_integer_walks():
+    """Float Q whose entries are multiples of u = 2^-k (MaxCut weights / 4, half-integer
+    RandomQUBOs): solved as the integer problem Q / u, energies u * E_int -- the float
+    path's exact answer, state and ties included."""
+    rng = np.random.default_rng(77)
+    cases = [gen.maxcut_gnp(18, 0.5, seed=5) / 4.0, gen.symmetric_int_qubo(17, 6).astype(np.float64) / 2.0,
+             rng.integers(-8, 9, (15, 15)) / 8.0, gen.tsp_int(4, seed=7) / 16.0]
+    for Q in cases:
+        p = engine.prepare(Q)
+        assert p.kind == "i32" and p.unit < 1.0
+        z = engine.solve_matrix(p, device="cpu")
+        f = engine.solve_matrix(engine.prepare(Q, lattice=False), device="cpu")
+        E, s = oracle.brute_f64(np.asarray(Q, dtype=np.float64))
+        assert (z.E, z.state) == (f.E, f.state) == (E, s) and isinstance(z.E, float)
+        _, mz, _ = engine.solve_matrix_minima(p, device="cpu", tie_B=10)
+        _, mf, _ = engine.solve_matrix_minima(engine.prepare(Q, lattice=False), device="cpu", tie_B=10)
+        assert np.array_equal(mz, mf)

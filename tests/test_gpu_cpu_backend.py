@@ -56,3 +56,15 @@ def test_parallel_qbf1_tie_key(gpu_ctx):
             assert bits(sol.best_X) == bits(whole.best_X) and sol.best_E == whole.best_E, (n, k)
     prob = RandomQUBO.init_random_instance(seed=9, N=22, dtype=np.int16)
     assert bits(Parallel(QB_F1, 4, ["cuda"]).solve(prob).best_X) == bits(QB_F1().solve(prob).best_X)
+
+
+def test_lattice_floats_on_the_gpu(gpu_ctx):
+    """Float Q on a power-of-two lattice (MaxCut weights / 4, halved int16) takes the integer
+    walks on the GPU too, with the float path's exact answer."""
+    for Q in (gen.maxcut_gnp(30, 0.5, seed=8) / 4.0, gen.symmetric_int_qubo(28, 9).astype(np.float64) / 2.0):
+        p = engine.prepare(Q)
+        z = engine.solve_matrix(p, device=0)
+        f = engine.solve_matrix(engine.prepare(Q, lattice=False), device=0)
+        c = engine.solve_matrix(p, device="cpu")
+        assert p.kind == "i32" and z.info["walk"] in ("pair16", "pair16_rebased")
+        assert (z.E, z.state) == (f.E, f.state) == (c.E, c.state)

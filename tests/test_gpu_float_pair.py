@@ -55,7 +55,8 @@ def test_packed_float_filter_matches_int_walk_large(gpu_ctx):
             Q = random_instance(rng, n, kind)
         for B in (None, 0):
             flush("float pair large", n, kind, B)
-            a = engine.solve_matrix(Q.astype(np.float64) / 4, tie_B=B, float_filter="packed")
+            a = engine.solve_matrix(engine.prepare(Q.astype(np.float64) / 4, lattice=False), tie_B=B,
+                                    float_filter="packed")
             b = engine.solve_matrix(Q, tie_B=B)
             assert a.info["walk"] == "float_pair16", a.info["walk"]
             assert (a.E * 4, a.state) == (b.E, b.state), (n, kind, B)

@@ -21,8 +21,14 @@ def test_path_selection():
     assert engine.prepare(c).kind == "i32"
     with pytest.raises(OverflowError):
         engine.prepare(np.array([[2 ** 62, 0], [0, 2 ** 62]], dtype=np.int64))
+    # multiples of one power of two: the integer walk on Q / u, energies u * E_int
     f = engine.prepare(np.eye(3, dtype=np.float32) * 0.5)
-    assert f.kind == "f64" and not f.integer
+    assert f.kind == "i32" and not f.integer and f.unit == 0.5 and f.matrix.tolist() == np.eye(3).tolist()
+    assert engine.prepare(np.eye(3, dtype=np.float32) * 0.5, lattice=False).kind == "f64"
+    assert engine.prepare(np.eye(3) * 0.1).kind == "f64"                       # 0.1: no short lattice
+    assert engine.prepare(np.full((4, 4), 2.0 ** 40 + 0.5)).kind == "f64"     # bound beyond int32
+    assert engine.lattice_bits(np.array([[0.75, 0.0], [-1.5, 2.0]])) == 2
+    assert engine.lattice_bits(np.zeros((2, 2))) == 0
     # integer-valued floats take the exact int32 walk (energies stay floats)
     g = engine.prepare(np.eye(3, dtype=np.float32))
     assert g.kind == "i32" and not g.integer
