@@ -561,31 +561,44 @@ def serve_leg(args, rates, count: int, burst: bool = True, sequential: bool = Tr
                        "P(n)~2^-(n-20), half int16 RandomQUBO [-127,127], half MaxCut G(n,0.5)",
            "rates": rows}
     if burst:
-        # capacity: every request submitted at once, in one submit_many call
-        with Server(max_n=40) as srv:
-            for p in preps[:64]:
-                srv.submit(p)
-            w = 0
-            while w < 64:
-                w += len(srv.poll(wait_ms=20))
-            st0 = srv.stats()
-            t0 = time.perf_counter()
-            ids = srv.submit_many(preps)
-            t_sub = time.perf_counter() - t0
-            got = {}
-            while len(got) < len(ids):
-                for r in srv.poll(wait_ms=20):
-                    got[r["id"]] = r
-            wall = time.perf_counter() - t0
-            st = srv.stats()
-        lat = np.array([got[i]["t_done_ms"] - got[i]["t_submit_ms"] for i in ids])
-        for j, i in enumerate(ids):
-            answers.setdefault(j, set()).add((got[i]["E"], got[i]["state"]))
-        out["burst"] = {"solves_per_s": len(ids) / wall, "submit_us_per_request": t_sub / len(ids) * 1e6,
-                        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                        "wall_ms": wall * 1e3, "steps": int(st["steps"] - st0["steps"]),
-                        "device_busy_ms": st["busy_ms"] - st0["busy_ms"]}
-        log(f"serve burst -> {out['burst']}")
+        # capacity: every request submitted at once -- from a buffer the client packed
+        # beforehand (submit_packed), and from the prepared problems (submit_many, which packs)
+        from qubo_lib_amd import _native
+        assert all(p.kind == "i32" and p.integer for p in preps)
+        t0 = time.perf_counter()
+        packed = _native.Context.pack([p.matrix for p in preps], np.int32)
+        pack_ms = (time.perf_counter() - t0) * 1e3
+        for form in ("packed", "problems"):
+            with Server(max_n=40) as srv:
+                for p in preps[:64]:
+                    srv.submit(p)
+                w = 0
+                while w < 64:
+                    w += len(srv.poll(wait_ms=20))
+                st0 = srv.stats()
+                t0 = time.perf_counter()
+                ids = list(srv.submit_packed(*packed)) if form == "packed" else srv.submit_many(preps)
+                t_sub = time.perf_counter() - t0
+                got = {}
+                while len(got) < len(ids):
+                    for r in srv.poll(wait_ms=20):
+                        got[r["id"]] = r
+                wall = time.perf_counter() - t0
+                st = srv.stats()
+            lat = np.array([got[i]["t_done_ms"] - got[i]["t_submit_ms"] for i in ids])
+            for j, i in enumerate(ids):
+                answers.setdefault(j, set()).add((got[i]["E"], got[i]["state"]))
+            busy = st["busy_ms"] - st0["busy_ms"]
+            row = {"solves_per_s": len(ids) / wall, "submit_us_per_request": t_sub / len(ids) * 1e6,
+                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                   "wall_ms": wall * 1e3, "steps": int(st["steps"] - st0["steps"]),
+                   "device_busy_ms": busy, "device_busy_frac": busy / (wall * 1e3)}
+            if form == "packed":
+                row["client_pack_ms"] = pack_ms
+                out["burst"] = row
+            else:
+                out["burst_from_problems"] = row
+            log(f"serve burst ({form}) -> {row}")
     if sequential:
         # sequential plain solves (one blocking launch each) on the same arrivals
         arr = arrivals * (base_rate / rates[0])

@@ -90,7 +90,9 @@ constexpr int MAX_GROUPS = 8;  // one persistent launch (and ticket) per inner-b
 // ------------------------------------------------------------------------------- prep
 // idx (nullable): problem index per workgroup; tickets (nullable): zeroed by workgroup 0;
 // staged (nullable): descriptors to install first (staged[b] -> insts[idx[b]]), so that a
-// batch of admissions needs one host-to-device copy and this one launch.
+// batch of admissions needs this one launch.  idx, staged and the source matrix I.Q may lie
+// in mapped host memory: each is read once (the matrix into LDS, where the table loops read
+// it), and nothing reads them after this kernel.
 static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uint32_t* idx,
                                                       unsigned long long* tickets,
                                                       const InstDev* staged) {
@@ -104,8 +106,15 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
     const int n = I.n, M = I.M;
     __shared__ double red_f[256];
     __shared__ long long red_i[256];
+    __shared__ uint32_t q_lds[MAX_N * MAX_N * 2];
+    {
+        const uint32_t* src = (const uint32_t*)I.Q;
+        const int words = n * n * (I.kind == KIND_I32 ? 1 : 2);
+        for (int w = tid; w < words; w += blockDim.x) q_lds[w] = src[w];
+        __syncthreads();
+    }
     if (I.kind == KIND_I32) {
-        const int32_t* Q = (const int32_t*)I.Q;
+        const int32_t* Q = (const int32_t*)q_lds;
         int32_t* W = (int32_t*)I.W;
         int32_t* d = (int32_t*)I.d;
         int32_t* dl = (int32_t*)I.delta;
@@ -152,7 +161,7 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         }
     } else {
         // FP64 tables for the exact re-walks, then their quantized int32 copies for the filter
-        const double* Q = (const double*)I.Q;
+        const double* Q = (const double*)q_lds;
         double* W = (double*)I.W;
         double* d = (double*)I.d;
         double* dld = (double*)I.delta;

@@ -50,7 +50,6 @@ static const Variant kVariants[] = {
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-constexpr int MAX_N = 56;
 constexpr int MAX_LANE_BITS = 37;   // tiles = 2^(H-6) < 2^31
 constexpr uint32_t MAX_CHUNKS = 1u << 22;
 

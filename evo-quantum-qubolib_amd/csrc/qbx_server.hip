@@ -1,19 +1,22 @@
 // qbx_server.hip -- continuous-batching server over the exhaustive walk kernels
 // (C ABI in include/qbx.h, "serving").
 //
-// A worker thread owns two streams for scheduling steps, one stream for admissions, and a
+// A worker thread owns three streams for scheduling steps, one stream for admissions, and a
 // pool of device slots (tables and reduction state of one request each).
 //   admission  oldest first, while a slot is free and the admitted work (sum of 2^n) stays
-//              within the budget; all requests admitted together are staged through one
-//              pinned buffer, one host-to-device copy and one prep launch.
+//              within the budget; all requests admitted together are staged in one mapped
+//              pinned buffer that their prep launch reads directly (no copy call: the
+//              runtime's host-to-device copy has blocked the calling thread for 6-7 ms now
+//              and then, and every request behind it waited).
 //   step       one finite persistent walk launch over chunks of every admitted request:
 //              requests with little work left go first (short requests are not held behind
 //              long ones), the oldest request always gets a share (no starvation).  A
 //              request may span several steps; its on-device reduction state (running
 //              minimum, candidate list, done counter) carries over and the chunk that
 //              finishes last writes its answer, exactly as in a single solve.
-//   overlap    steps alternate between the two streams, so the tail of one step overlaps
-//              the next; a step's result read-back waits for the previous step too.
+//   overlap    steps rotate over the three streams, up to three in flight: the host work
+//              between one step's completion and the next launch overlaps the two steps
+//              still queued; a step's result read-back waits for the previous step too.
 //   completion a request is done once every step holding its chunks has completed; its
 //              answer is read from the step's copy of the result array.
 // Nothing on the device waits for the host: every launch is finite.
@@ -26,6 +29,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -45,7 +49,8 @@ struct Req {
     int kind = KIND_I32;
     int n = 0;
     int tie_B = -1;
-    std::vector<char> Q;   // n*n elements of the kind's type
+    std::shared_ptr<char[]> qblk;   // matrices of the submit call (shared by its requests)
+    size_t q_off = 0, q_bytes = 0;   // this request's n*n elements of the kind's type
     double t_submit = 0, t_admit = 0;
     int slot = -1;
     Plan plan;
@@ -63,6 +68,9 @@ struct SlotLayout {
 };
 
 constexpr int kStageSlots = 2;
+// Steps in flight: the worker's host work for a step (completions, admission, chunk choice,
+// launch) overlaps the device work of the other queued steps.
+constexpr int kDepth = 3;
 
 }  // namespace
 
@@ -76,30 +84,33 @@ struct qbx_server {
     WalkTable walk;
     SlotLayout L;
     int blocks_per_cu[kNumVariants];
-    hipStream_t st[2] = {nullptr, nullptr};
+    hipStream_t st[kDepth] = {};
     hipStream_t st_admit = nullptr;
-    hipEvent_t ev_step[2] = {nullptr, nullptr};
-    hipEvent_t ev_start[2] = {nullptr, nullptr};
+    hipEvent_t ev_step[kDepth] = {};
+    hipEvent_t ev_start[kDepth] = {};
     hipEvent_t ev_admit = nullptr;
+    hipEvent_t ev_ref = nullptr;        // recorded at start: steps' times are measured from it
+    double busy_end = 0.0;              // end of the union of step intervals so far (ms from ev_ref)
     hipEvent_t ev_stage[kStageSlots] = {nullptr, nullptr};
     bool stage_used[kStageSlots] = {false, false};
     // device
     char* d_slots = nullptr;
     InstDev* d_inst = nullptr;
     ResultDev* d_results = nullptr;
-    char* d_stage[kStageSlots] = {nullptr, nullptr};
-    TileRange* d_ranges[2] = {nullptr, nullptr};
-    unsigned long long* d_tickets[2] = {nullptr, nullptr};
+    TileRange* d_ranges[kDepth] = {};
+    unsigned long long* d_tickets[kDepth] = {};
     size_t stage_bytes = 0;
     // pinned host
-    char* h_stage[kStageSlots] = {nullptr, nullptr};
-    TileRange* h_ranges[2] = {nullptr, nullptr};
-    ResultDev* h_results[2] = {nullptr, nullptr};
+    char* h_stage[kStageSlots] = {nullptr, nullptr};   // mapped, coherent: read by qbx_prep
+    char* m_stage[kStageSlots] = {nullptr, nullptr};   // their device addresses
+    TileRange* h_ranges[kDepth] = {};
+    ResultDev* h_results[kDepth] = {};
     // shared with submitters
     std::mutex m;
     std::condition_variable cv_in, cv_out;
     std::deque<Req*> incoming;
     std::deque<qbx_request_result_t> done;
+    std::vector<Req*> req_pool;         // finished requests, reused by submitters
     bool stop = false;
     uint64_t next_id = 1;
     std::thread worker;
@@ -108,6 +119,7 @@ struct qbx_server {
     std::deque<Req*> waiting;
     std::vector<Req*> active;
     std::vector<int> free_slots;
+    std::vector<Req*> retired;          // finished since the last publish_done
     double inflight_work = 0;
     int stage_next = 0;
     // stats
@@ -130,7 +142,8 @@ size_t stage_need(const qbx_server* S) {
            align256(q) * S->opts.max_inflight;
 }
 
-void finish_request(qbx_server* S, Req* r, const ResultDev* res) {
+// The result record of a finished request, appended to `out` (published by publish_done).
+void finish_request(qbx_server* S, Req* r, const ResultDev* res, std::vector<qbx_request_result_t>& done) {
     qbx_request_result_t out;
     memset(&out, 0, sizeof out);
     out.id = r->id;
@@ -153,13 +166,28 @@ void finish_request(qbx_server* S, Req* r, const ResultDev* res) {
         }
     }
     out.t_done_ms = S->now_ms();
+    done.push_back(out);
+    S->retired.push_back(r);
+}
+
+// Hands finished requests to the pollers: one lock and one wake-up for all of them.  Their
+// Req records go back to the pool (allocation and release stay off the request path).
+void publish_done(qbx_server* S, std::vector<qbx_request_result_t>& done) {
+    if (done.empty()) return;
+    constexpr size_t kPoolMax = 16384;
     {
         std::lock_guard<std::mutex> g(S->m);
-        S->done.push_back(out);
+        S->done.insert(S->done.end(), done.begin(), done.end());
+        while (!S->retired.empty() && S->req_pool.size() < kPoolMax) {
+            S->req_pool.push_back(S->retired.back());
+            S->retired.pop_back();
+        }
     }
+    for (Req* r : S->retired) delete r;
+    S->retired.clear();
+    S->n_completed += done.size();
     S->cv_out.notify_all();
-    S->n_completed++;
-    delete r;
+    done.clear();
 }
 
 int admit(qbx_server* S, std::vector<Req*>& batch) {
@@ -167,13 +195,12 @@ int admit(qbx_server* S, std::vector<Req*>& batch) {
     S->stage_next = (b + 1) % kStageSlots;
     if (S->stage_used[b]) HIP_TRY(hipEventSynchronize(S->ev_stage[b]));   // staging buffer free again
     char* h = S->h_stage[b];
-    char* dv = S->d_stage[b];
+    char* dv = S->m_stage[b];
     const size_t off_idx = align256(sizeof(InstDev) * S->opts.max_inflight);
     const size_t off_q = off_idx + align256(4 * (size_t)S->opts.max_inflight);
-    const size_t qstride = align256((size_t)S->opts.max_n * S->opts.max_n * 8);
     InstDev* hi = (InstDev*)h;
     uint32_t* hidx = (uint32_t*)(h + off_idx);
-    size_t used = off_q;
+    size_t used = off_q;    // matrices packed back to back (256-B aligned), only what they hold
     for (size_t j = 0; j < batch.size(); ++j) {
         Req* r = batch[j];
         const Plan& p = r->plan;
@@ -182,7 +209,7 @@ int admit(qbx_server* S, std::vector<Req*>& batch) {
         memset(&I, 0, sizeof I);
         I.n = p.n; I.B = p.B; I.L = p.L; I.M = p.M; I.kind = r->kind; I.status = 0;
         I.tiles = p.tiles; I.tpc = p.tpc; I.chunks = p.chunks; I.expect = p.chunks; I.flags = plan_flags(p, 0);
-        I.Q = dv + off_q + j * qstride;
+        I.Q = dv + used;
         I.W = base + S->L.off_W;
         I.d = base + S->L.off_d;
         I.delta = base + S->L.off_delta;
@@ -191,12 +218,10 @@ int admit(qbx_server* S, std::vector<Req*>& batch) {
         I.slots = (SlotDev*)(base + S->L.off_slots);
         I.result = S->d_results + r->slot;
         hidx[j] = (uint32_t)r->slot;
-        memcpy(h + off_q + j * qstride, r->Q.data(), r->Q.size());
-        used = off_q + (j + 1) * qstride;
-        r->Q.clear();
-        r->Q.shrink_to_fit();
+        memcpy(h + used, r->qblk.get() + r->q_off, r->q_bytes);
+        used = align256(used + r->q_bytes);
+        r->qblk.reset();
     }
-    HIP_TRY(hipMemcpyAsync(dv, h, used, hipMemcpyHostToDevice, S->st_admit));
     hipLaunchKernelGGL(qbx_prep, dim3((unsigned)batch.size()), dim3(256), 0, S->st_admit, S->d_inst,
                        (const uint32_t*)(dv + off_idx), (unsigned long long*)nullptr, (const InstDev*)dv);
     HIP_TRY(hipGetLastError());
@@ -209,29 +234,37 @@ int admit(qbx_server* S, std::vector<Req*>& batch) {
 
 // Marks every request whose chunks have all run in completed steps as done.
 int complete_step(qbx_server* S, int64_t k) {
-    const int s = (int)(k & 1);
+    const int s = (int)(k % kDepth);
     HIP_TRY(hipEventSynchronize(S->ev_step[s]));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, S->ev_start[s], S->ev_step[s]) == hipSuccess)
-        S->busy_ms.store(S->busy_ms.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
+    // device busy time: the union of the steps' intervals (steps on different streams overlap,
+    // and complete in launch order)
+    float t_beg = 0.f, t_end = 0.f;
+    if (hipEventElapsedTime(&t_beg, S->ev_ref, S->ev_start[s]) == hipSuccess &&
+        hipEventElapsedTime(&t_end, S->ev_ref, S->ev_step[s]) == hipSuccess) {
+        const double b = std::max<double>(t_beg, S->busy_end);
+        if (t_end > b) S->busy_ms.store(S->busy_ms.load(std::memory_order_relaxed) + (t_end - b), std::memory_order_relaxed);
+        S->busy_end = std::max<double>(S->busy_end, t_end);
+    }
     std::vector<Req*> keep;
     keep.reserve(S->active.size());
+    std::vector<qbx_request_result_t> done;
     for (Req* r : S->active) {
         if (r->next_chunk >= r->plan.chunks && r->last_step <= k) {
             const ResultDev res = S->h_results[s][r->slot];
             S->free_slots.push_back(r->slot);
             S->inflight_work -= r->work;
-            finish_request(S, r, &res);
+            finish_request(S, r, &res, done);
         } else {
             keep.push_back(r);
         }
     }
     S->active.swap(keep);
+    publish_done(S, done);
     return QBX_OK;
 }
 
 int launch_step(qbx_server* S, int64_t k, std::vector<std::pair<Req*, uint32_t>>& take) {
-    const int s = (int)(k & 1);
+    const int s = (int)(k % kDepth);
     // group by kernel variant; one range per request
     std::stable_sort(take.begin(), take.end(), [](const std::pair<Req*, uint32_t>& a, const std::pair<Req*, uint32_t>& b) {
         return a.first->plan.variant < b.first->plan.variant;
@@ -273,8 +306,9 @@ int launch_step(qbx_server* S, int64_t k, std::vector<std::pair<Req*, uint32_t>>
                            (const TileRange*)S->d_ranges[s] + G.first, G.cnt, G.total, S->d_tickets[s] + g, 0);
         HIP_TRY(hipGetLastError());
     }
-    // the result copy must also see chunks of earlier steps finished on the other stream
-    if (k > 0) HIP_TRY(hipStreamWaitEvent(st, S->ev_step[s ^ 1], 0));
+    // the result copy must also see chunks of earlier steps finished on the other streams:
+    // step k - 1's copy waited for step k - 2's, and so on
+    if (k > 0) HIP_TRY(hipStreamWaitEvent(st, S->ev_step[(k - 1) % kDepth], 0));
     HIP_TRY(hipMemcpyAsync(S->h_results[s], S->d_results, sizeof(ResultDev) * S->opts.max_inflight,
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(S->ev_step[s], st));
@@ -335,20 +369,23 @@ void worker_main(qbx_server* S) {
             stopping = S->stop;
         }
         if (S->worker_rc != QBX_OK) {   // device failure: fail everything still queued
-            for (Req* r : S->waiting) { r->status = S->worker_rc; finish_request(S, r, nullptr); }
+            std::vector<qbx_request_result_t> failed;
+            for (Req* r : S->waiting) { r->status = S->worker_rc; finish_request(S, r, nullptr, failed); }
             S->waiting.clear();
-            for (Req* r : S->active) { r->status = S->worker_rc; finish_request(S, r, nullptr); }
+            for (Req* r : S->active) { r->status = S->worker_rc; finish_request(S, r, nullptr, failed); }
             S->active.clear();
+            publish_done(S, failed);
             if (stopping) break;
             continue;
         }
         // admission: oldest first, slots and work budget permitting
         std::vector<Req*> batch;
+        std::vector<qbx_request_result_t> rejected;
         while (!S->waiting.empty()) {
             Req* r = S->waiting.front();
             if (r->status != QBX_OK) {
                 S->waiting.pop_front();
-                finish_request(S, r, nullptr);
+                finish_request(S, r, nullptr, rejected);
                 continue;
             }
             if (S->free_slots.empty()) break;
@@ -361,12 +398,13 @@ void worker_main(qbx_server* S) {
             batch.push_back(r);
             S->active.push_back(r);
         }
+        publish_done(S, rejected);
         if (!batch.empty()) {
             const int rc = admit(S, batch);
             if (rc != QBX_OK) { S->worker_rc = rc; S->worker_msg = "admission failed"; continue; }
         }
         // make room for the next step's buffers
-        if (inflight.size() >= 2) {
+        if (inflight.size() >= (size_t)kDepth) {
             const int rc = complete_step(S, inflight.front());
             inflight.pop_front();
             if (rc != QBX_OK) { S->worker_rc = rc; S->worker_msg = "step failed"; }
@@ -421,11 +459,11 @@ int server_alloc(qbx_server* S) {
     HIP_TRY(hipMalloc((void**)&S->d_results, sizeof(ResultDev) * K));
     HIP_TRY(hipMemset(S->d_results, 0, sizeof(ResultDev) * K));
     for (int b = 0; b < kStageSlots; ++b) {
-        HIP_TRY(hipMalloc((void**)&S->d_stage[b], S->stage_bytes));
-        HIP_TRY(hipHostMalloc((void**)&S->h_stage[b], S->stage_bytes, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&S->h_stage[b], S->stage_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+        HIP_TRY(hipHostGetDevicePointer((void**)&S->m_stage[b], S->h_stage[b], 0));
         HIP_TRY(hipEventCreateWithFlags(&S->ev_stage[b], hipEventDisableTiming));
     }
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kDepth; ++s) {
         HIP_TRY(hipStreamCreateWithFlags(&S->st[s], hipStreamNonBlocking));
         HIP_TRY(hipEventCreate(&S->ev_step[s]));
         HIP_TRY(hipEventCreate(&S->ev_start[s]));
@@ -437,6 +475,8 @@ int server_alloc(qbx_server* S) {
     HIP_TRY(hipStreamCreateWithFlags(&S->st_admit, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&S->ev_admit, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(S->ev_admit, S->st_admit));
+    HIP_TRY(hipEventCreate(&S->ev_ref));
+    HIP_TRY(hipEventRecord(S->ev_ref, S->st_admit));
     for (int v = 0; v < kNumVariants; ++v) {
         int b = 0;
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kVariants[v].fn, 64, 0));
@@ -447,16 +487,15 @@ int server_alloc(qbx_server* S) {
 }
 
 void server_free(qbx_server* S) {
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kDepth; ++s) {
         if (S->st[s]) (void)hipStreamSynchronize(S->st[s]);
     }
     if (S->st_admit) (void)hipStreamSynchronize(S->st_admit);
     for (int b = 0; b < kStageSlots; ++b) {
-        if (S->d_stage[b]) (void)hipFree(S->d_stage[b]);
         if (S->h_stage[b]) (void)hipHostFree(S->h_stage[b]);
         if (S->ev_stage[b]) (void)hipEventDestroy(S->ev_stage[b]);
     }
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kDepth; ++s) {
         if (S->d_ranges[s]) (void)hipFree(S->d_ranges[s]);
         if (S->d_tickets[s]) (void)hipFree(S->d_tickets[s]);
         if (S->h_ranges[s]) (void)hipHostFree(S->h_ranges[s]);
@@ -466,33 +505,49 @@ void server_free(qbx_server* S) {
         if (S->st[s]) (void)hipStreamDestroy(S->st[s]);
     }
     if (S->ev_admit) (void)hipEventDestroy(S->ev_admit);
+    if (S->ev_ref) (void)hipEventDestroy(S->ev_ref);
     if (S->st_admit) (void)hipStreamDestroy(S->st_admit);
     if (S->d_slots) (void)hipFree(S->d_slots);
     if (S->d_inst) (void)hipFree(S->d_inst);
     if (S->d_results) (void)hipFree(S->d_results);
 }
 
-// A request for problem Q of size n, planned and copied (not yet queued).
-template <class T>
-int make_request(qbx_server* S, int kind, const T* Q, int n, int tie_B, int index, Req** out) {
+// The plan of a request of size n, checked against the server's limits.
+int plan_request(qbx_server* S, int kind, int n, int tie_B, int index, Plan& p) {
     if (n < 1 || n > S->opts.max_n)
         return set_err(QBX_EINVAL, "request %d: n=%d outside [1, %d] (server max_n)", index, n, S->opts.max_n);
-    Req* r = new Req();
-    r->kind = kind;
-    r->n = n;
-    r->tie_B = tie_B;
-    r->work = ldexp(1.0, n);
     qbx_opts_t o = {tie_B, S->opts.walk_bits, -1, 0};
-    const int rc = plan_problem(kind, n, o, PLAN_BATCH, index, r->plan, &S->walk);
-    if (rc != QBX_OK) { delete r; return rc; }
-    if (r->plan.chunks > S->L.max_chunks) {
-        delete r;
-        return set_err(QBX_EINVAL, "request %d: walk_bits too small for the slot size", index);
-    }
-    r->Q.resize(sizeof(T) * (size_t)n * n);
-    memcpy(r->Q.data(), Q, r->Q.size());
-    *out = r;
+    const int rc = plan_problem(kind, n, o, PLAN_BATCH, index, p, &S->walk);
+    if (rc != QBX_OK) return rc;
+    if (p.chunks > S->L.max_chunks) return set_err(QBX_EINVAL, "request %d: walk_bits too small for the slot size", index);
     return QBX_OK;
+}
+
+// count fresh Req records: from the pool of finished ones, else newly allocated.
+void new_requests(qbx_server* S, Req** out, int count) {
+    int got = 0;
+    {
+        std::lock_guard<std::mutex> g(S->m);
+        while (got < count && !S->req_pool.empty()) {
+            out[got++] = S->req_pool.back();
+            S->req_pool.pop_back();
+        }
+    }
+    for (int i = 0; i < got; ++i) *out[i] = Req();
+    for (; got < count; ++got) out[got] = new Req();
+}
+
+// Request r for the matrix at blk + off (not yet queued).
+void fill_request(Req* r, int kind, const std::shared_ptr<char[]>& blk, size_t off, int tie_B, const Plan& plan,
+                  size_t elem) {
+    r->kind = kind;
+    r->n = plan.n;
+    r->tie_B = tie_B;
+    r->work = ldexp(1.0, plan.n);
+    r->plan = plan;
+    r->qblk = blk;
+    r->q_off = off;
+    r->q_bytes = elem * (size_t)plan.n * plan.n;
 }
 
 // Queue requests (ids in order, under one lock, one wake-up of the worker), or none.
@@ -519,26 +574,58 @@ int enqueue_requests(qbx_server* S, Req** reqs, int count, uint64_t* ids) {
 template <class T>
 int submit(qbx_server* S, int kind, const T* Q, int n, int tie_B, uint64_t* id) {
     if (!S || !Q || !id) return set_err(QBX_EINVAL, "null argument");
-    Req* r = nullptr;
-    const int rc = make_request<T>(S, kind, Q, n, tie_B, 0, &r);
+    Plan p;
+    const int rc = plan_request(S, kind, n, tie_B, 0, p);
     if (rc != QBX_OK) return rc;
+    const size_t bytes = sizeof(T) * (size_t)n * n;
+    std::shared_ptr<char[]> blk(new char[bytes]);
+    memcpy(blk.get(), Q, bytes);
+    Req* r;
+    new_requests(S, &r, 1);
+    fill_request(r, kind, blk, 0, tie_B, p, sizeof(T));
     return enqueue_requests(S, &r, 1, id);
 }
 
+// A burst: every request is checked first (all are queued, or none), then they are copied
+// and queued in chunks (one matrix block per chunk), so the worker starts on the first
+// chunk while later ones are still being copied.
 template <class T>
 int submit_many(qbx_server* S, int kind, int count, const T* Qpacked, const int64_t* offsets, const int32_t* ns,
                 int tie_B, uint64_t* ids) {
     if (!S || count < 0 || (count > 0 && (!Qpacked || !offsets || !ns || !ids)))
         return set_err(QBX_EINVAL, "null argument");
-    std::vector<Req*> reqs((size_t)count, nullptr);
+    // one plan per size (the plan depends on n alone here), checked for every request
+    Plan plans[MAX_N + 1];
+    bool planned[MAX_N + 1] = {};
     for (int i = 0; i < count; ++i) {
-        const int rc = make_request<T>(S, kind, Qpacked + offsets[i], ns[i], tie_B, i, &reqs[i]);
-        if (rc != QBX_OK) {
-            for (int j = 0; j < i; ++j) delete reqs[j];
-            return rc;
-        }
+        if (offsets[i] < 0) return set_err(QBX_EINVAL, "request %d: negative offset", i);
+        const int n = ns[i];
+        if (n >= 1 && n <= MAX_N && planned[n]) continue;
+        Plan p;
+        const int rc = plan_request(S, kind, n, tie_B, i, p);
+        if (rc != QBX_OK) return rc;
+        plans[n] = p;
+        planned[n] = true;
     }
-    return count ? enqueue_requests(S, reqs.data(), count, ids) : QBX_OK;
+    constexpr int kChunk = 256;
+    Req* reqs[kChunk];
+    for (int i0 = 0; i0 < count; i0 += kChunk) {
+        const int c = std::min(kChunk, count - i0);
+        size_t total = 0;
+        for (int i = 0; i < c; ++i) total += sizeof(T) * (size_t)ns[i0 + i] * ns[i0 + i];
+        std::shared_ptr<char[]> blk(new char[total]);
+        new_requests(S, reqs, c);
+        size_t off = 0;
+        for (int i = 0; i < c; ++i) {
+            const Plan& p = plans[ns[i0 + i]];
+            fill_request(reqs[i], kind, blk, off, tie_B, p, sizeof(T));
+            memcpy(blk.get() + off, Qpacked + offsets[i0 + i], reqs[i]->q_bytes);
+            off += reqs[i]->q_bytes;
+        }
+        const int rc = enqueue_requests(S, reqs, c, ids + i0);
+        if (rc != QBX_OK) return rc;
+    }
+    return QBX_OK;
 }
 
 }  // namespace
@@ -552,7 +639,7 @@ int qbx_server_start(qbx_ctx_t* ctx, const qbx_server_opts_t* opts, qbx_server_t
     S->ctx = ctx;
     S->walk.clear();
     S->device = ctx->device;
-    qbx_server_opts_t o = {40, 256, 1099511627776.0 /* 2^40 */, 17179869184.0 /* 2^34 */, -1, 0};
+    qbx_server_opts_t o = {40, 512, 1099511627776.0 /* 2^40 */, 17179869184.0 /* 2^34 */, -1, 0};
     if (opts) {
         if (opts->max_n > 0) o.max_n = opts->max_n;
         if (opts->max_inflight > 0) o.max_inflight = opts->max_inflight;
@@ -634,6 +721,8 @@ int qbx_server_stop(qbx_server_t* S) {
     std::string msg = S->worker_msg;
     server_free(S);
     for (Req* r : S->incoming) delete r;
+    for (Req* r : S->req_pool) delete r;
+    for (Req* r : S->retired) delete r;
     delete S;
     if (rc != QBX_OK) return set_err(rc, "server worker failed: %s", msg.c_str());
     return QBX_OK;

@@ -113,6 +113,7 @@ QBX_HD uint32_t block_key(uint32_t c, int SBk, uint32_t pmaskS) {
 // Leading dimension of the bit-ordered W table: rows are zero padded to 64 entries so
 // that unrolled code may read W[J][0..63] for any J < n without bounds checks.
 constexpr int LDW = 64;
+constexpr int MAX_N = 56;   // variables of the largest problem
 
 // W[a][b] in bit order from the row-major, variable-ordered source Q (int32 or double).
 template <class T, class Acc>

@@ -257,7 +257,7 @@ typedef struct qbx_server qbx_server_t;
 
 typedef struct {
     int32_t max_n;            /* largest n admitted (<= 56); sizes each slot; default 40 */
-    int32_t max_inflight;     /* device slots (admitted requests); default 256 */
+    int32_t max_inflight;     /* device slots (admitted requests); default 512 */
     double work_budget;       /* max sum of 2^n over admitted requests; default 2^40 */
     double step_states;       /* states scheduled per step launch; default 2^34 */
     int32_t walk_bits;        /* -1: per-n default */
@@ -279,7 +279,7 @@ typedef struct {
 
 typedef struct {
     uint64_t submitted, completed, steps, chunks, admissions;
-    double busy_ms;           /* sum of step device times (HIP events) */
+    double busy_ms;           /* device time covered by steps: the union of their intervals (HIP events) */
 } qbx_server_stats_t;
 
 int qbx_server_start(qbx_ctx_t *ctx, const qbx_server_opts_t *opts, qbx_server_t **out);
@@ -287,7 +287,9 @@ int qbx_server_start(qbx_ctx_t *ctx, const qbx_server_opts_t *opts, qbx_server_t
 int qbx_server_submit_i32(qbx_server_t *s, const int32_t *Q, int n, int tie_B, uint64_t *id);
 int qbx_server_submit_f64(qbx_server_t *s, const double *Q, int n, int tie_B, uint64_t *id);
 /* A burst of requests in one call (packed as for qbx_batch_*): ids[i] for problem i, in
- * order; all are queued, or none (the error names the first bad request). */
+ * order.  Every request is checked before any is queued, so a bad one queues none (the
+ * error names it); they are then copied and queued 256 at a time, and the worker starts on
+ * the first of them while the rest are still being copied. */
 int qbx_server_submit_many_i32(qbx_server_t *s, int count, const int32_t *Qpacked, const int64_t *offsets,
                                const int32_t *ns, int tie_B, uint64_t *ids);
 int qbx_server_submit_many_f64(qbx_server_t *s, int count, const double *Qpacked, const int64_t *offsets,

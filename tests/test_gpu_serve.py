@@ -149,3 +149,24 @@ def test_server_packed_float_filter_requests(gpu_ctx):
         if q.shape[0] <= 24:
             E, s = oracle.brute_f64(q)
             assert r["state"] == s and abs(r["E"] - E) <= 1e-13 * np.abs(q).sum()
+
+
+def test_server_packed_burst_matches_individual_solves(gpu_ctx):
+    """submit_packed: a burst the client packed itself (no per-request Python work) gives
+    every request its own solve's answer, ids in order.  A second burst and single submits
+    on the same server (request records and staging buffers reused) do too."""
+    from qubo_lib_amd import _native
+    with Server(max_n=40) as srv:
+        for seed, count in ((5, 600), (6, 900)):
+            reqs = gen.serving_requests(count, 20000.0, seed=seed)
+            preps = [engine.prepare(q) for _, q, _ in reqs]
+            packed = _native.Context.pack([p.matrix for p in preps], np.int32)
+            ids = list(srv.submit_packed(*packed))
+            assert ids == sorted(ids)
+            singles = preps[::97]
+            ids += [srv.submit(p) for p in singles]
+            got = drain(srv, len(ids))
+            for p, i in zip(preps + singles, ids):
+                alone = engine.solve_matrix(p)
+                assert (got[int(i)]["E"], got[int(i)]["state"]) == (alone.E, alone.state), (seed, i)
+                assert isinstance(got[int(i)]["E"], int)
