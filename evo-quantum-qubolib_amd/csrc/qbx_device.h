@@ -571,7 +571,7 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                     const uint64_t lam = lam0 + (uint64_t)lane;
                     const bool active = H >= 6 || lam < (1ull << H);
                     const uint64_t c0 = rec ? wall_clock64() : 0ull;
-                    const LaneOutF64 o = walk_lane_qrp<M, OMAX>(Wq, dq, Wpc, W, d, n, L, I.B, active ? lam : lam0,
+                    const LaneOutF64 o = walk_lane_qrp<M, OMAX, !REC>(Wq, dq, Wpc, W, d, n, L, I.B, active ? lam : lam0,
                                                                 dlp, dsrc, dld, I.qscale, I.qmargin, I.qshift,
                                                                 I.slack, wb, qc + lane, qmn + lane, 64, nullptr);
                     if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }
@@ -586,7 +586,7 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                     const uint64_t lam = lam0 + (uint64_t)lane;
                     const bool active = H >= 6 || lam < (1ull << H);
                     const uint64_t c0 = rec ? wall_clock64() : 0ull;
-                    const LaneOutF64 o = walk_lane_q32<M, OMAX>(Wq, dq, W, d, n, L, I.B, active ? lam : lam0, dlq,
+                    const LaneOutF64 o = walk_lane_q32<M, OMAX, !REC>(Wq, dq, W, d, n, L, I.B, active ? lam : lam0, dlq,
                                                                 dld, I.qscale, I.qmargin, wb, qc + lane, qmn + lane,
                                                                 64);
                     if (active && (o.E < E || (o.E == E && o.rank < rank))) { E = o.E; rank = o.rank; }

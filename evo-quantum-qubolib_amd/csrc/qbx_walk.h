@@ -1302,6 +1302,166 @@ QBX_HD void q_compact(QBuf& b, QLane& q, const QCtx<M>& x, bool walk, double& be
     b.count = keep;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+    return ((uint64_t)readlane_u32((uint32_t)(v >> 32), l) << 32) | readlane_u32((uint32_t)v, l);
+}
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Lane j's column of the base state (inner bits 0) of the block whose bits above the inner
+// block are hs: d_j plus W_aj over the set high bits a (a >= M), all of them for an inner
+// column j < M (its field f_j), those below j for j >= M (bit j's share of E0 when set).
+// Row a is one coalesced load of its LDW padded entries; no load past row n - 1.
+template <class T, class Acc, int M>
+__device__ __forceinline__ Acc block_column_wave(const T* __restrict__ W, const T* __restrict__ d, int n,
+                                                 uint64_t hs, int lane) {
+    Acc acc = lane < n ? (Acc)d[lane] : (Acc)0;
+    const int lim = lane < M ? n : lane;
+#pragma unroll 8
+    for (int a = M; a < n; ++a) {
+        const T w = W[a * LDW + lane];
+        if (((hs >> (a - M)) & 1ull) && a < lim) acc = acc + (Acc)w;
+    }
+    return acc;
+}
+
+// Energy of inner state y of a block from its base energy E0, fields f and the inner
+// couplings W_lk (l < k < M): E0 + sum over set bits k of (f_k + sum over set l < k of W_lk).
+template <class T, class Acc, int M>
+__device__ __forceinline__ Acc inner_energy(Acc E0, const Acc* f, const T* __restrict__ W, uint32_t y) {
+    Acc E = E0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        Acc e = f[k];
+#pragma unroll
+        for (int l = 0; l < k; ++l) e = ((y >> l) & 1u) ? e + (Acc)W[l * LDW + k] : e;
+        E = ((y >> k) & 1u) ? E + e : E;
+    }
+    return E;
+}
+#endif
+
+// The lane-end flush of the filtered walks (q_compact with `walk`), done by the whole wave:
+// every passing record of every lane is tested and walked by the 64 lanes together.  A
+// lane's own flush walks its records one after another, its block rebuilt serially from
+// scratch, while the wave's other lanes wait: one record took 0.1 ms, a sixth of a 2^34-state
+// solve, in the wave holding the answer.  Here lane j sums column j of the tables over the
+// block's set high bits, the 2^M inner states are split over the lanes and evaluated
+// directly, and wave reductions give the block's exact int32 minimum (PACKED records' test
+// before their FP64 walk) and its FP64 (E, rank) minimum, which every lane folds into its
+// best.  Records are pruned by the least lane minimum and threshold over the wave (each is
+// a bound from a real state, whichever lane found it).  Every lane must call it, and the
+// lanes' results are pooled: not for walks that keep per-lane minima.  Host builds flush
+// each lane's records in turn (q_compact).
+template <int M, bool PACKED>
+QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uint64_t& bestR) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    q_compact<M, PACKED>(b, q, x, false, bestE, bestR);    // drop what no longer passes
+    const int lane = (int)__lane_id();
+    const int O = x.L - M;
+    int32_t laneMin = wave_min_i32(q.laneMin);
+    const int32_t thrB = wave_min_i32(q.thrB);
+    auto thr_of = [&](int32_t lm) {
+        const int32_t loc = sat_add_i32(lm, q.two_qm);
+        int32_t t = thrB < loc ? thrB : loc;
+        if (PACKED) t = t < QRP_THR_MAX ? t : QRP_THR_MAX;
+        return t;
+    };
+    int32_t thr = thr_of(laneMin);
+    uint64_t pending = __ballot(b.count > 0);
+    while (pending) {
+        const int s = __builtin_ctzll(pending);
+        pending &= pending - 1ull;
+        const int cnt = (int)readlane_u32((uint32_t)b.count, s);
+        const uint64_t lam = readlane_u64(x.lam, s);
+        const uint64_t p = lane_prefix(lam, x.L, x.B);
+        const uint64_t laneRank = shl64(lam, x.L);
+        const int SBk = (x.B - M) < 0 ? 0 : ((x.B - M) > O ? O : (x.B - M));
+        const int parp = parity64(p);
+        const uint32_t pmaskS = (parp && O > SBk) ? (lowmask32(O - SBk) << SBk) : 0u;
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t c = readlane_u32(b.c[k * b.stride], s);
+            const int32_t m = (int32_t)readlane_u32((uint32_t)b.m[k * b.stride], s);
+            if (m > thr) continue;
+            for (uint32_t h = 0; h < (PACKED ? 2u : 1u); ++h) {
+                uint32_t cf = c;
+                if (PACKED) {
+                    const uint32_t cp = c >> 2;
+                    if (!((c >> h) & 1u)) continue;
+                    cf = (cp << 1) | (h ^ (cp & 1u));
+                }
+                const uint32_t gc = cf ^ (cf >> 1);
+                const uint64_t hs = (p << O) | (uint64_t)gc;
+                if (PACKED) {
+                    // the int32 filter's exact test: the block's least quantized energy
+                    const uint32_t colq = block_column_wave<int32_t, uint32_t, M>(x.Wq, x.dq, x.n, hs, lane);
+                    const bool set = lane >= M && lane < x.n && ((hs >> (lane - M)) & 1ull);
+                    uint32_t e0 = set ? colq : 0u;
+                    for (int o = 32; o > 0; o >>= 1) e0 += (uint32_t)__shfl_xor((int32_t)e0, o);
+                    uint32_t fq[M];
+#pragma unroll
+                    for (int j = 0; j < M; ++j) fq[j] = readlane_u32(colq, j);
+                    int32_t mn = 0x7fffffff;
+#pragma unroll
+                    for (int t = 0; t < ((1 << M) + 63) / 64; ++t) {
+                        const uint32_t y = (uint32_t)lane + 64u * (uint32_t)t;
+                        if (y < (1u << M)) {
+                            const int32_t E = (int32_t)inner_energy<int32_t, uint32_t, M>(e0, fq, x.Wq, y);
+                            mn = E < mn ? E : mn;
+                        }
+                    }
+                    mn = wave_min_i32(mn);
+                    if (mn < laneMin) {
+                        laneMin = mn;
+                        thr = thr_of(laneMin);
+                    }
+                    if (mn > thr) continue;
+                }
+                // the block in FP64: its states' energies and ranks, the wave's least pair
+                const double col = block_column_wave<double, double, M>(x.W, x.d, x.n, hs, lane);
+                const bool set = lane >= M && lane < x.n && ((hs >> (lane - M)) & 1ull);
+                double e0 = set ? col : 0.0;
+                for (int o = 32; o > 0; o >>= 1) e0 += __shfl_xor(e0, o);
+                double f[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) f[j] = __shfl(col, j);
+                const uint64_t rblk = laneRank | ((uint64_t)block_key(cf, SBk, pmaskS) << M);
+                const int par = parp ^ parity32(gc);
+                double E = __builtin_inf();
+                uint64_t r = ~0ull;
+#pragma unroll
+                for (int t = 0; t < ((1 << M) + 63) / 64; ++t) {
+                    const uint32_t y = (uint32_t)lane + 64u * (uint32_t)t;
+                    if (y < (1u << M)) {
+                        const double Ey = inner_energy<double, double, M>(e0, f, x.W, y);
+                        const uint64_t ry = rblk | rank_lo<M>(y, x.B, par);
+                        if (Ey < E || (Ey == E && ry < r)) { E = Ey; r = ry; }
+                    }
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double E2 = __shfl_xor(E, o);
+                    const uint64_t r2 = __shfl_xor(r, o);
+                    if (E2 < E || (E2 == E && r2 < r)) { E = E2; r = r2; }
+                }
+                if (E < bestE || (E == bestE && r < bestR)) { bestE = E; bestR = r; }
+            }
+        }
+    }
+    b.count = 0;
+#else
+    q_compact<M, PACKED>(b, q, x, true, bestE, bestR);
+#endif
+}
+
 // One block of an unrolled group of the filtered walk: static transition of the int32
 // state, int32 block minimum, lane minimum and filter update, candidate record.
 template <int M, int OMAX, int J>
@@ -1356,7 +1516,7 @@ struct HostBound {
 //     Eq(x) <= s E(x) + qm0 <= s E(xq) + qm0 <= mq + 2 qm0.
 // Both thresholds only decrease during the walk, so a block dropped at any time fails the
 // final filter too.  The lane publishes (mq + qm0 + 1) / s, an upper bound on E(xq).
-template <int M, int OMAX, class Bound>
+template <int M, int OMAX, bool COOP = false, class Bound>
 QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
                                 const double* __restrict__ W, const double* __restrict__ d,
                                 int n, int L, int B, uint64_t lam, const int32_t* dlq,
@@ -1412,7 +1572,8 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
         const bool up = (((c0 ^ (c0 >> 1)) >> i) & 1u) != 0u;
         trans_dyn_i32<M, OMAX>(s, Wq + (M + i) * LDW, i, up);
     }
-    q_compact<M>(b, q, x, true, bestE, bestR);
+    if constexpr (COOP) q_flush_wave<M, false>(b, q, x, bestE, bestR);
+    else q_compact<M>(b, q, x, true, bestE, bestR);
     LaneOutF64 out;
     out.E = bestE;
     out.rank = bestR;
@@ -1614,7 +1775,7 @@ QBX_HD void group_qrp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ Wq, 
 // Walk one lane of a float problem with the packed filter (tables: Wq, dq and the coarse
 // packed Wpc, dlpc of shift sh; slack = PAIR_BOUND - D_c); same contract and FP64 re-walks
 // as walk_lane_q32.  groups_int32 (nullable) counts the groups walked in int32.
-template <int M, int OMAX, class Bound>
+template <int M, int OMAX, bool COOP = false, class Bound>
 QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
                                 const uint32_t* __restrict__ Wpc, const double* __restrict__ W,
                                 const double* __restrict__ d, int n, int L, int B, uint64_t lam,
@@ -1709,7 +1870,8 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
         trans_dyn_rp<M, OMAX>(s, Wq + (M + 1 + i) * LDW, i, up);
     }
     wave_priority_by_progress(nblk, nblk);
-    q_compact<M, true>(b, q, x, true, bestE, bestR);
+    if constexpr (COOP) q_flush_wave<M, true>(b, q, x, bestE, bestR);
+    else q_compact<M, true>(b, q, x, true, bestE, bestR);
     LaneOutF64 out;
     out.E = bestE;
     out.rank = bestR;

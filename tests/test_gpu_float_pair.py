@@ -94,3 +94,31 @@ def test_float_filter_choice(gpu_ctx):
         assert len({(r.E, r.state) for r in runs}) == 1
     with pytest.raises(ValueError):
         engine.solve_matrix(small, float_filter="fp16")
+
+
+@pytest.mark.parametrize("flt", ["packed", "int32"])
+def test_lane_end_flush_with_many_records_and_exact_ties(gpu_ctx, flt):
+    """The lane-end flush, which the wave does together over every lane's records, under
+    load: integer structure (many blocks within the filter's margin of the optimum) with
+    a tiny perturbation, and integer-valued matrices beyond the int32 bound, whose FP64
+    energies are exact, so that equal minima in different lanes are decided by the rank.
+    Answers equal the FP64 brute force and the host build's lane-by-lane flush."""
+    rng = np.random.default_rng(4242)
+    for trial in range(10):
+        n = int(rng.integers(14, 22))
+        base = rng.integers(-2, 3, (n, n)).astype(np.float64)
+        if trial % 2:
+            Q = base * 2.0 ** 33                       # exact FP64 ties, f64 path
+        else:
+            Q = base + rng.standard_normal((n, n)) * 1e-8
+        p = engine.prepare(Q, lattice=False)
+        assert p.kind == "f64"
+        B = [None, 0, n][trial % 3]
+        E_ref, s_ref = oracle.brute_f64(Q, B)
+        host = engine.solve_matrix(p, device="cpu", tie_B=B, float_filter=flt)
+        assert host.state == s_ref and close(host.E, E_ref, Q), (trial, n, B)
+        for L in sorted({min(n, 9), min(n, 12), n}):
+            flush("flush", flt, trial, n, B, L)
+            got = engine.solve_matrix(p, tie_B=B, walk_bits=L, inner_bits=7, float_filter=flt)
+            assert got.state == s_ref and close(got.E, E_ref, Q), (flt, trial, n, B, L)
+            assert (got.E, got.state) == (host.E, host.state) or trial % 2 == 0, (flt, trial, n, B, L)
