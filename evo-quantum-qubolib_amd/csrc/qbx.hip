@@ -65,6 +65,8 @@ struct Layout {
     int gen_dtype = 0;
     int64_t gen_low = 0, gen_high = 0;        // RandomQUBO bounds
     size_t off_rng = 0;
+    // float problems large enough for qbx_seed (one launch after qbx_prep seeds them)
+    bool seed = false;
     // per-reference-prefix records of a single solve (qbx_solve_minima_*)
     bool minima = false;
     uint64_t nprefix = 0;                      // 2^(n - B)
@@ -142,6 +144,7 @@ int make_layout(qbx_ctx* c, int kind, int count, const int32_t* ns, const qbx_op
         if (rc) return rc;
         p.off_Q = off;
         off = align256(off + lay.esize * (size_t)p.n * p.n);
+        if (kind == KIND_F64 && p.n >= SEED_MIN_N) lay.seed = true;
     }
     // staged from the host: [instances | ranges | Q data], or [instances | ranges | generator
     // records] when the matrices are generated on the device
@@ -299,6 +302,11 @@ int enqueue(qbx_ctx* c, hipStream_t stream, const Layout& lay, char* dv, const c
     hipLaunchKernelGGL(qbx_prep, dim3(lay.count), dim3(256), 0, stream, (InstDev*)(dv + lay.off_inst),
                        (const uint32_t*)nullptr, dticket, (const InstDev*)nullptr);
     HIP_TRY(hipGetLastError());
+    if (lay.seed) {
+        hipLaunchKernelGGL(qbx_seed, dim3(lay.count), dim3(64 * SEED_WAVES), 0, stream, (InstDev*)(dv + lay.off_inst),
+                           (const uint32_t*)nullptr);
+        HIP_TRY(hipGetLastError());
+    }
     for (size_t g = 0; g < lay.groups.size(); ++g) {
         const Group& G = lay.groups[g];
         const int v = G.variant;

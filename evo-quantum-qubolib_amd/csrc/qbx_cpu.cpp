@@ -204,12 +204,14 @@ struct AtomicBound {
         while (e < cur && !v->compare_exchange_weak(cur, e, std::memory_order_relaxed)) {
         }
     }
+    void note(double) {}
 };
 
 // No shared bound: every lane finds its own exact minimum (per-prefix records).
 struct NoBound {
     double refresh() const { return __builtin_inf(); }
     void publish(double) {}
+    void note(double) {}
 };
 
 template <int M, int OMAX, class Bound>

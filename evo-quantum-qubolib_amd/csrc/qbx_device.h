@@ -87,6 +87,106 @@ struct TileRange {
 
 constexpr int MAX_GROUPS = 8;  // one persistent launch (and ticket) per inner-bits variant
 
+// ----------------------------------------------------- float filter: a seeded bound
+// The float filters walk fastest when their threshold is tight from the first block: with
+// no bound, every lane records blocks until its own minimum and the problem's shared bound
+// have come down (measured at n = 34: the first three of 32 groups took 3.4x a later one).
+// So qbx_seed seeds the shared bound with the energy of a good state: 16 one-flip descents
+// (best improvement, FP32), one per wave, each lane holding one variable's flip gain, the
+// wave's best move found by a DPP min-reduction of (gain, variable) keys; the least result
+// is re-evaluated in FP64.  Any state's energy bounds the answer from above; the seed adds
+// 2^-40 of the energy bound (far above the rounding of any FP64 energy sum of at most
+// 56 x 57 / 2 terms), so that the walk's own evaluation of that state never lies above it
+// and its tile still commits as a candidate.
+// the device library's wave reductions (DPP sequences; the f32 sum is declared by the HIP
+// headers only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES)
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_add_f32(float);
+
+constexpr int SEED_MIN_N = 26;   // smaller problems walk too briefly to gain from a seed
+constexpr int SEED_WAVES = 16;   // descents per problem (one workgroup of 16 waves)
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// order-preserving key of a float's bits, low 6 bits replaced by c (an index < 64)
+__device__ __forceinline__ uint32_t seed_key(float g, uint32_t c) {
+    const uint32_t u = __float_as_uint(g);
+    return (((u & 0x80000000u) ? ~u : (u | 0x80000000u)) & ~63u) | c;
+}
+
+// qbx_seed: one workgroup of SEED_WAVES waves per problem (idx as for qbx_prep, which it
+// follows on the same stream).  It reads the prepared bit-ordered FP64 tables W and d, and
+// sets the problem's reduction key to the seed; problems it does not serve keep the empty
+// key: integer problems, float problems out of range or below SEED_MIN_N, and shares of a
+// solve (expect < chunks: the seed's state may lie outside the share).
+static __global__ __launch_bounds__(64 * SEED_WAVES) void qbx_seed(InstDev* insts, const uint32_t* idx) {
+    InstDev& I = insts[idx ? idx[blockIdx.x] : blockIdx.x];
+    const int n = I.n;
+    if (I.kind != KIND_F64 || I.status != 0 || n < SEED_MIN_N || I.expect != I.chunks) return;
+    const double* W = (const double*)I.W;
+    const double* d = (const double*)I.d;
+    const int c = (int)__lane_id();            // this lane's variable (bit position c)
+    const int wv = (int)(threadIdx.x >> 6);
+    __shared__ float wf[MAX_N * 64];           // w_ab in FP32 (row a, column b), d_a on the diagonal
+    for (int i = threadIdx.x; i < MAX_N * 64; i += blockDim.x) {
+        const int a = i >> 6, b = i & 63;
+        wf[i] = (a < n && b < n) ? (float)(a == b ? d[a] : W[a * LDW + b]) : 0.0f;
+    }
+    __syncthreads();
+    // the descent of this wave: x wave-uniform, lane c holds g_c, the energy change of
+    // flipping variable c (+1 for the lanes past n: never a move)
+    uint64_t x = splitmix64((uint64_t)wv * 0x100000001b3ull + (uint64_t)n) & lowmask64(n);
+    float h = c < n ? wf[c * 64 + c] : 0.0f;    // d_c + sum_{j != c, x_j} w_jc
+    for (uint64_t r = x; r; r &= r - 1ull) {
+        const int j = __builtin_ctzll(r);
+        if (j != c) h += wf[j * 64 + c];
+    }
+    const bool xc = (x >> c) & 1ull;
+    float E = __ockl_wfred_add_f32(xc ? 0.5f * (wf[c * 64 + c] + h) : 0.0f);
+    float g = c < n ? (xc ? -h : h) : 1.0f;
+    for (int step = 0; step < 2 * n; ++step) {
+        const uint32_t m = __ockl_wfred_min_u32(seed_key(g, (uint32_t)c));
+        if (m >= 0x80000000u) break;           // no move lowers the energy
+        const int bi = (int)(m & 63u);
+        E += __uint_as_float(~(m & ~63u));      // the gain, to 2^-17 relative (a heuristic)
+        const float sg = ((x >> bi) & 1ull) ? -1.0f : 1.0f;   // the change of x_bi
+        x ^= 1ull << bi;
+        const float w = wf[bi * 64 + c];
+        g = (c == bi) ? -g : g + (((x >> c) & 1ull) ? -sg : sg) * w;
+    }
+    __shared__ float best_E[SEED_WAVES];
+    __shared__ uint64_t best_x[SEED_WAVES];
+    if (c == 0) {
+        best_E[wv] = E == E ? E : __builtin_inff();
+        best_x[wv] = x;
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    // the least of the descents, then its exact energy: lane c adds x_c (d_c + sum_{b < c,
+    // x_b} W_bc), reading column c of row b (coalesced)
+    float eb = best_E[0];
+    uint64_t xb = best_x[0];
+    for (int k = 1; k < SEED_WAVES; ++k)
+        if (best_E[k] < eb) {
+            eb = best_E[k];
+            xb = best_x[k];
+        }
+    double v = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < n; ++b) {      // same trip count on every lane: the loads pipeline
+        const double w = W[b * LDW + c];
+        v += (b < c && ((xb >> b) & 1ull)) ? w : 0.0;
+    }
+    v = (c < n && ((xb >> c) & 1ull)) ? v + d[c] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (c == 0 && v - v == 0.0) I.red->minkey = key_of_f64(v + I.fbound * 0x1p-40);
+}
+
 // ------------------------------------------------------------------------------- prep
 // idx (nullable): problem index per workgroup; tickets (nullable): zeroed by workgroup 0;
 // staged (nullable): descriptors to install first (staged[b] -> insts[idx[b]]), so that a
@@ -217,22 +317,53 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
             dlq[t] = (int32_t)trie_from_table<int32_t, long long>(Wq, M, t);
         // the rows the packed filter's shift choice reads (inner, pair and group bits), in
-        // LDS: thread 0 walks them serially
+        // LDS; thread 0 takes the estimate, then one thread per candidate shift tests it
         __shared__ int32_t wq_head[(7 + 1 + GROUP_BITS) * LDW];
+        __shared__ double sh_S, sh_rms[8], sh_din[8];
+        __shared__ int sh_start;
+        __shared__ int sh_ok[32];
+        __shared__ long long sh_D[32];
         const int head_rows = n < M + 1 + GROUP_BITS ? n : M + 1 + GROUP_BITS;
         for (int idx = tid; idx < head_rows * LDW; idx += blockDim.x) wq_head[idx] = Wq[idx];
+        // finite energies whose quantized bound fits int32 (q_scale guarantees it)
+        const bool in_range = ok && red_i[0] <= 0x7ffffffeLL;
+        const bool try_packed = in_range && !(I.flags & QBX_FLAG_FLOAT_INT32_FILTER) && I.L > M + 1;
+        __syncthreads();
+        if (try_packed && tid < M) qrp_shift_term(wq_head, dq, n, M, tid, &sh_rms[tid], &sh_din[tid]);
+        __syncthreads();
+        if (tid == 0) {
+            double S = 0.0, Din = 0.0;
+            for (int k = 0; k < M; ++k) {
+                S += sh_rms[k];
+                Din += sh_din[k];
+            }
+            sh_S = S;
+            sh_start = try_packed ? qrp_shift_first(S, Din, M) : -1;
+        }
+        __syncthreads();
+        const int s0 = sh_start;
+        if (tid < 31) {
+            int64_t D = 0;
+            sh_ok[tid] = (s0 >= 0 && tid >= s0 && qrp_shift_ok(wq_head, M, I.L, tid, sh_S, &D)) ? 1 : 0;
+            sh_D[tid] = D;
+        }
         __syncthreads();
         if (tid == 0) {
             I.fbound = fb;
             I.qscale = scale;
             I.qmargin = q_margin(n);
             I.bound = red_i[0];
-            // finite energies whose quantized bound fits int32 (q_scale guarantees it)
-            I.status = (ok && red_i[0] <= 0x7ffffffeLL) ? 0 : QBX_ERANGE;
-            // the packed filter wherever a coarse scale leaves its guard room
+            I.status = in_range ? 0 : QBX_ERANGE;
+            // the packed filter wherever a coarse scale leaves its guard room: the smallest
+            // shift from the estimate up that passes (what qrp_shift finds serially)
+            int sh = -1;
             int64_t D = 0;
-            const int sh = (I.status == 0 && !(I.flags & QBX_FLAG_FLOAT_INT32_FILTER))
-                               ? qrp_shift(wq_head, dq, n, M, I.L, &D) : -1;
+            for (int s = s0 < 0 ? 31 : s0; s <= 30; ++s)
+                if (sh_ok[s]) {
+                    sh = s;
+                    D = sh_D[s];
+                    break;
+                }
             I.path = sh >= 0 ? QBX_PATH_FLOAT_PAIR16 : QBX_PATH_FLOAT;
             I.qshift = sh >= 0 ? sh : 0;
             I.slack = sh >= 0 ? (int32_t)(PAIR_BOUND - D) : 0;
@@ -405,6 +536,27 @@ __device__ __forceinline__ bool next_tile(const TileRange* __restrict__ ranges, 
 }
 
 // ----------------------------------------------------------------------- int32 walk
+// K uniform table entries into VGPRs (they are too many for the scalar registers), 16 bytes
+// per load where K allows: src is 16-byte aligned (every table section is), and z is a
+// lane-dependent zero that keeps the compiler from treating the values as uniform.
+template <int K>
+__device__ __forceinline__ void load_uniform_vgprs(uint32_t* dst, const uint32_t* __restrict__ src, uint32_t z) {
+    if constexpr (K % 4 == 0) {
+        const uint4* s4 = (const uint4*)src;
+#pragma unroll
+        for (int k = 0; k < K / 4; ++k) {
+            const uint4 v = s4[k + z];
+            dst[4 * k] = v.x;
+            dst[4 * k + 1] = v.y;
+            dst[4 * k + 2] = v.z;
+            dst[4 * k + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) dst[k] = src[k + z];
+    }
+}
+
 __device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uni64(uint64_t v) {
     return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
@@ -457,8 +609,7 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
             // offset keeps them out of the scalar register file)
             uint32_t dlp[1 << M];
             const uint32_t* psrc = (const uint32_t*)(dsrc + (1 << M));
-#pragma unroll
-            for (int k = 0; k < (1 << M); ++k) dlp[k] = psrc[k + (lane & zero)];
+            load_uniform_vgprs<1 << M>(dlp, psrc, lane & zero);
             for (uint32_t tile = t0; tile < t1; ++tile) {
                 const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
                 const uint64_t c0 = rec ? wall_clock64() : 0ull;
@@ -480,8 +631,7 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
                 rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dsrc);
         } else {
             int32_t dl[1 << M];
-#pragma unroll
-            for (int k = 0; k < (1 << M); ++k) dl[k] = dsrc[k + (lane & zero)];
+            load_uniform_vgprs<1 << M>((uint32_t*)dl, (const uint32_t*)dsrc, lane & zero);
             if (I.status == 0) {
                 for (uint32_t tile = t0; tile < t1; ++tile) {
                     const uint64_t lam = (uint64_t)tile * 64u + (uint64_t)lane;
@@ -506,9 +656,11 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
 
 // ------------------------------------------------------------------ float-filter walk
 // Running upper bound shared by every wave of one problem: the problem's reduction key
-// (an exact energy of some state, or empty).  publish() takes the wave minimum of the
-// lanes' exact bests and lowers the key when it improved on what this wave last offered.
-// All 64 lanes must call it together (it shuffles across the wave).
+// (an upper bound on the energy of some state, or empty; qbx_prep may seed it).
+// publish() takes the wave minimum of the lanes' exact bests and lowers the key when it
+// improves on the least value this wave has offered or read (note()), so that waves which
+// hold nothing better than the shared bound send no atomic.  All 64 lanes must call
+// publish() together (it shuffles across the wave).
 struct WaveBound {
     unsigned long long* minkey;
     double offered;
@@ -516,6 +668,7 @@ struct WaveBound {
     __device__ double refresh() const {
         return live ? energy_of_key(ld_relaxed((const uint64_t*)minkey)) : __builtin_inf();
     }
+    __device__ void note(double b) { offered = b < offered ? b : offered; }
     __device__ void publish(double e) {
         if (!live) return;
         for (int off = 32; off > 0; off >>= 1) {
@@ -564,8 +717,7 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                 const uint32_t* Wpc = (const uint32_t*)(Wq + (size_t)n * LDW);
                 const uint32_t* psrc = (const uint32_t*)(dsrc + (1 << M));
                 uint32_t dlp[1 << M];
-#pragma unroll
-                for (int k = 0; k < (1 << M); ++k) dlp[k] = psrc[k + (lane & zero)];
+                load_uniform_vgprs<1 << M>(dlp, psrc, lane & zero);
                 for (uint32_t tile = t0; tile < t1; ++tile) {
                     const uint64_t lam0 = (uint64_t)tile * 64u;
                     const uint64_t lam = lam0 + (uint64_t)lane;
@@ -579,8 +731,7 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                 }
             } else {
                 int32_t dlq[1 << M];
-#pragma unroll
-                for (int k = 0; k < (1 << M); ++k) dlq[k] = dsrc[k + (lane & zero)];
+                load_uniform_vgprs<1 << M>((uint32_t*)dlq, (const uint32_t*)dsrc, lane & zero);
                 for (uint32_t tile = t0; tile < t1; ++tile) {
                     const uint64_t lam0 = (uint64_t)tile * 64u;
                     const uint64_t lam = lam0 + (uint64_t)lane;

@@ -225,6 +225,13 @@ int admit(qbx_server* S, std::vector<Req*>& batch) {
     hipLaunchKernelGGL(qbx_prep, dim3((unsigned)batch.size()), dim3(256), 0, S->st_admit, S->d_inst,
                        (const uint32_t*)(dv + off_idx), (unsigned long long*)nullptr, (const InstDev*)dv);
     HIP_TRY(hipGetLastError());
+    bool seed = false;
+    for (const Req* r : batch) seed = seed || (r->kind == KIND_F64 && r->plan.n >= SEED_MIN_N);
+    if (seed) {   // float requests large enough: their shared bound starts from a seed
+        hipLaunchKernelGGL(qbx_seed, dim3((unsigned)batch.size()), dim3(64 * SEED_WAVES), 0, S->st_admit, S->d_inst,
+                           (const uint32_t*)(dv + off_idx));
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(S->ev_admit, S->st_admit));
     HIP_TRY(hipEventRecord(S->ev_stage[b], S->st_admit));
     S->stage_used[b] = true;

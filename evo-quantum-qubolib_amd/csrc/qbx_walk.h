@@ -51,6 +51,15 @@
 
 namespace qbx {
 
+// Tables read through the constant address space: they are written by an earlier launch
+// and only read by the walks, so where the address is wave-uniform the device reads them
+// with scalar loads into SGPRs (scalar cache, short latency, no VGPRs).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QBX_CONST_AS __attribute__((address_space(4)))
+#else
+#define QBX_CONST_AS
+#endif
+
 // Path counters of the host build (the CPU tests check that every path of the float
 // filter is exercised); compiled out of the device build.
 #ifdef QBX_EMU_STATS
@@ -442,7 +451,11 @@ QBX_HD void group_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const 
 // are the same for the 64 lanes of a wave: their contributions are accumulated once, from
 // wave-uniform values (scalar registers on the device), and only the low bits cost
 // per-lane work.
-template <int M, int OMAX>
+// SROWS: read the low lane bits' rows through the constant address space (scalar loads on
+// the device).  Measured on MI355X: the float filters gain from it (n = 34 float solve
+// 0.550 -> 0.541 ms), the integer walks lose (n = 40: 24.66 -> 25.19 ms), so only the
+// float walks ask for it.
+template <int M, int OMAX, bool SROWS = false>
 QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* __restrict__ d,
                            int n, int L, int B, uint64_t lam) {
     const int H = n - L;
@@ -455,6 +468,52 @@ QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
     // contributions of the set high lane bits (uniform)
     uint32_t Ec = 0;
     int32_t fc[M], uc[OMAX], lc[6];
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (H > Hlo) {
+        // Wave-cooperative (every lane of the wave calls this when H > 6): lane c sums
+        // column c of W over the set high bits a, in full for the columns below them (the
+        // fields of the inner, outer and low lane bits) and over a < c for a high bit c
+        // itself (its share of E0).  One coalesced row load per high bit, all independent,
+        // instead of a serial chain of scalar loads.  Integer sums: the same values.
+        const int c = (int)__lane_id();
+        const int hcol = L + Hlo;
+        uint32_t acc = 0u;
+        // batches of 8 unconditional row loads (rows clamped into the table), then masked
+        // adds: every load of a batch is in flight before the first is used
+        for (int j0 = Hlo; j0 < H; j0 += 8) {
+            uint32_t w[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int j = j0 + t < H ? j0 + t : H - 1;
+                w[t] = (uint32_t)W[(L + j) * LDW + c];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int j = j0 + t, a = L + j;
+                const bool on = j < H && ((p_hi >> (j - Hlo)) & 1ull) && (c < hcol || a < c);
+                acc += w[t] & (0u - (uint32_t)on);
+            }
+        }
+        const uint32_t v = (c < n ? (uint32_t)d[c] : 0u) + acc;
+        const bool in_hi = c >= hcol && c < n && ((p_hi >> (c - hcol)) & 1ull);
+        uint32_t e = in_hi ? v : 0u;
+        for (int o = 32; o > 0; o >>= 1) e += (uint32_t)__shfl_xor((int)e, o);
+        Ec = (uint32_t)__builtin_amdgcn_readfirstlane((int)e);
+#pragma unroll
+        for (int k = 0; k < M; ++k) fc[k] = __builtin_amdgcn_readlane((int)v, k);
+#pragma unroll
+        for (int q = 0; q < OMAX; ++q) uc[q] = (q < O) ? __builtin_amdgcn_readlane((int)v, M + q) : 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) lc[i] = (i < Hlo) ? __builtin_amdgcn_readlane((int)v, L + i) : 0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; ++k) fc[k] = d[k];
+#pragma unroll
+        for (int q = 0; q < OMAX; ++q) uc[q] = (q < O) ? d[M + q] : 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) lc[i] = (i < Hlo) ? d[L + i] : 0;
+    }
+#else
 #pragma unroll
     for (int k = 0; k < M; ++k) fc[k] = d[k];
 #pragma unroll
@@ -476,6 +535,7 @@ QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
 #pragma unroll
         for (int i = 0; i < 6; ++i) lc[i] += (i < Hlo) ? Wa[L + i] : 0;
     }
+#endif
     // per-lane part: the low lane bits (lc[i] already holds d + the high-bit couplings)
     uint32_t E0 = Ec;
 #pragma unroll
@@ -487,16 +547,19 @@ QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
         if (i >= Hlo) break;
         const uint32_t mi = 0u - ((p_lo >> i) & 1u);
         const int a = L + i;
-        const int32_t* Wa = W + a * LDW;
+        // (the same row, read with scalar loads or not)
+        const QBX_CONST_AS int32_t* Was = (const QBX_CONST_AS int32_t*)(uintptr_t)(W + a * LDW);
+        const int32_t* Wav = W + a * LDW;
+        auto Wa = [&](int k) -> int32_t { return SROWS ? Was[k] : Wav[k]; };
         uint32_t e = (uint32_t)lc[i];
 #pragma unroll
         for (int i2 = i + 1; i2 < 6; ++i2)
-            if (i2 < Hlo) e += (uint32_t)Wa[L + i2] & (0u - ((p_lo >> i2) & 1u));
+            if (i2 < Hlo) e += (uint32_t)Wa(L + i2) & (0u - ((p_lo >> i2) & 1u));
         E0 += e & mi;
 #pragma unroll
-        for (int k = 0; k < M; ++k) s.f[k] += (int32_t)((uint32_t)Wa[k] & mi);
+        for (int k = 0; k < M; ++k) s.f[k] += (int32_t)((uint32_t)Wa(k) & mi);
 #pragma unroll
-        for (int q = 0; q < OMAX; ++q) s.u[q] += (int32_t)((uint32_t)Wa[M + q] & mi);
+        for (int q = 0; q < OMAX; ++q) s.u[q] += (int32_t)((uint32_t)Wa(M + q) & mi);
     }
     s.E0 = (int32_t)E0;
 }
@@ -828,11 +891,6 @@ QBX_HD void load_wrow_rp(WRowRP<M>& r, const int32_t* __restrict__ W, const uint
 // The same row read through the constant address space: the tables are written by an
 // earlier launch and only read by the walks, and the address is wave-uniform, so on the
 // device these become scalar loads into SGPRs (scalar cache, short latency, no VGPRs).
-#if defined(__HIP_DEVICE_COMPILE__)
-#define QBX_CONST_AS __attribute__((address_space(4)))
-#else
-#define QBX_CONST_AS
-#endif
 template <int M, int I>
 QBX_HD void load_wrow_rp_uniform(WRowRP<M>& r, const int32_t* __restrict__ W, const uint32_t* __restrict__ Wp) {
     constexpr int J = M + 1 + I;
@@ -1004,11 +1062,11 @@ QBX_HD void group_rp_int32(LaneRP<M, OMAX>& s, const int32_t* __restrict__ W, co
 }
 
 // Set-up: the int32 walk's first block (pair bit clear) and the pair bit's field.
-template <int M, int OMAX>
+template <int M, int OMAX, bool SROWS = false>
 QBX_HD void lane_setup_rp(LaneRP<M, OMAX>& s, const int32_t* __restrict__ W, const int32_t* __restrict__ d,
                           int n, int L, int B, uint64_t lam) {
     LaneI32<M, OMAX> a;
-    lane_setup_i32<M, OMAX>(a, W, d, n, L, B, lam);
+    lane_setup_i32<M, OMAX, SROWS>(a, W, d, n, L, B, lam);
     s.E0 = a.E0;
     s.up = a.u[0];
 #pragma unroll
@@ -1326,12 +1384,52 @@ __device__ __forceinline__ Acc block_column_wave(const T* __restrict__ W, const 
                                                  uint64_t hs, int lane) {
     Acc acc = lane < n ? (Acc)d[lane] : (Acc)0;
     const int lim = lane < M ? n : lane;
-#pragma unroll 8
-    for (int a = M; a < n; ++a) {
-        const T w = W[a * LDW + lane];
-        if (((hs >> (a - M)) & 1ull) && a < lim) acc = acc + (Acc)w;
+    // batches of 8 unconditional row loads (rows clamped into the table), then the adds
+    for (int a0 = M; a0 < n; a0 += 8) {
+        T w[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) w[t] = W[(a0 + t < n ? a0 + t : n - 1) * LDW + lane];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int a = a0 + t;
+            const bool on = a < n && ((hs >> (a - M)) & 1ull) && a < lim;
+            acc = on ? acc + (Acc)w[t] : acc;
+        }
     }
     return acc;
+}
+
+// Both of a block's base columns at once, the quantized (int32) and the FP64 one, in one
+// pass over the rows: every row load of the two tables is issued before the first is
+// needed, so a record's exact int32 test and its FP64 walk share one memory latency.
+// The FP64 sum is block_column_wave's, in the same order.
+template <int M>
+__device__ __forceinline__ void block_columns_wave(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
+                                                   const double* __restrict__ W, const double* __restrict__ d,
+                                                   int n, uint64_t hs, int lane, uint32_t& colq, double& col) {
+    uint32_t aq = lane < n ? (uint32_t)dq[lane] : 0u;
+    double ad = lane < n ? d[lane] : 0.0;
+    const int lim = lane < M ? n : lane;
+    // batches of 8 unconditional row loads (rows clamped into the tables), then masked adds
+    for (int a0 = M; a0 < n; a0 += 8) {
+        uint32_t wq[8];
+        double wd[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int a = a0 + t < n ? a0 + t : n - 1;
+            wq[t] = (uint32_t)Wq[a * LDW + lane];
+            wd[t] = W[a * LDW + lane];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int a = a0 + t;
+            const bool on = a < n && ((hs >> (a - M)) & 1ull) && a < lim;
+            aq += wq[t] & (0u - (uint32_t)on);
+            ad = on ? ad + wd[t] : ad;
+        }
+    }
+    colq = aq;
+    col = ad;
 }
 
 // Energy of inner state y of a block from its base energy E0, fields f and the inner
@@ -1365,6 +1463,10 @@ __device__ __forceinline__ Acc inner_energy(Acc E0, const Acc* f, const T* __res
 template <int M, bool PACKED>
 QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uint64_t& bestR) {
 #if defined(__HIP_DEVICE_COMPILE__)
+    // latency-bound and short: top issue priority, so that the other wave of the SIMD,
+    // still walking, does not hold it up (the flush of the wave holding the answer ends
+    // the solve)
+    __builtin_amdgcn_s_setprio(3);
     q_compact<M, PACKED>(b, q, x, false, bestE, bestR);    // drop what no longer passes
     const int lane = (int)__lane_id();
     const int O = x.L - M;
@@ -1401,9 +1503,12 @@ QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uin
                 }
                 const uint32_t gc = cf ^ (cf >> 1);
                 const uint64_t hs = (p << O) | (uint64_t)gc;
+                uint32_t colq = 0u;
+                double col;
+                if (PACKED) block_columns_wave<M>(x.Wq, x.dq, x.W, x.d, x.n, hs, lane, colq, col);
+                else col = block_column_wave<double, double, M>(x.W, x.d, x.n, hs, lane);
                 if (PACKED) {
                     // the int32 filter's exact test: the block's least quantized energy
-                    const uint32_t colq = block_column_wave<int32_t, uint32_t, M>(x.Wq, x.dq, x.n, hs, lane);
                     const bool set = lane >= M && lane < x.n && ((hs >> (lane - M)) & 1ull);
                     uint32_t e0 = set ? colq : 0u;
                     for (int o = 32; o > 0; o >>= 1) e0 += (uint32_t)__shfl_xor((int32_t)e0, o);
@@ -1427,7 +1532,6 @@ QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uin
                     if (mn > thr) continue;
                 }
                 // the block in FP64: its states' energies and ranks, the wave's least pair
-                const double col = block_column_wave<double, double, M>(x.W, x.d, x.n, hs, lane);
                 const bool set = lane >= M && lane < x.n && ((hs >> (lane - M)) & 1ull);
                 double e0 = set ? col : 0.0;
                 for (int o = 32; o > 0; o >>= 1) e0 += __shfl_xor(e0, o);
@@ -1501,6 +1605,7 @@ struct HostBound {
     double b;
     QBX_HD_MEMBER double refresh() const { return b; }
     QBX_HD_MEMBER void publish(double e) { b = e < b ? e : b; }
+    QBX_HD_MEMBER void note(double) {}
 };
 
 // Walk one lane of a float problem: int32 filter over quantized tables (Wq, dq, dlq in
@@ -1525,12 +1630,14 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
     static_assert(OMAX >= GROUP_BITS, "OMAX must cover the unrolled transition bits");
     const int O = L - M;
     LaneI32<M, OMAX> s;
-    lane_setup_i32<M, OMAX>(s, Wq, dq, n, L, B, lam);
+    wave_priority_by_progress(0u, 1u);   // the set-up at the first quarter's priority
+    lane_setup_i32<M, OMAX, true>(s, Wq, dq, n, L, B, lam);
 
     QCtx<M> x;
     x.set(W, d, dld, n, L, B, lam);
 
     double bound = src.refresh();
+    src.note(bound);
     double bestE = __builtin_inf();
     uint64_t bestR = ~0ull;
     QLane q;
@@ -1561,6 +1668,7 @@ QBX_HD LaneOutF64 walk_lane_q32(const int32_t* __restrict__ Wq, const int32_t* _
             }
             src.publish(mine);
             const double b2 = src.refresh();
+            src.note(b2);
             bound = b2 < bound ? b2 : bound;
             const double bb = bestE < bound ? bestE : bound;
             q.thrB = q_threshold(bb, scale, qm);
@@ -1646,28 +1754,60 @@ QBX_HD int64_t qrp_guard_D(const int32_t* Wq, int M, int L, int sh) {
 // Shift of the packed float filter: the smallest sh (finest coarse scale) for which the
 // guard constant leaves room for the expected inner fields (the estimate of rp_worth_it,
 // on the quantized tables), or -1 (then the int32 filter runs).  *D_out: D_c at sh.
-QBX_HD int qrp_shift(const int32_t* Wq, const int32_t* dq, int n, int M, int L, int64_t* D_out) {
-    if (L <= M + 1) return -1;
-    double S = 0.0, Din = 0.0;
-    for (int k = 0; k < M; ++k) {
-        double mean = dq[k], var = 0.0;
-        for (int j = M; j < n; ++j) {
-            const double w = Wq[k * LDW + j];
-            mean += 0.5 * w;
-            var += 0.25 * w * w;
-        }
-        S += sqrt(mean * mean + var);
-        for (int l = k + 1; l < M; ++l) Din += fabs((double)Wq[k * LDW + l]);
-        for (int i = 0; i <= GROUP_BITS && M + i < n; ++i) Din += fabs((double)Wq[(M + i) * LDW + k]);
+// Split up so that the device can compute the terms and test the candidate shifts in
+// parallel: qrp_shift_term() gives inner bit k's share of the estimate (the root mean
+// square of its field, and its coupling sum, an exact integer), qrp_shift_first() the
+// first shift tried (just below the estimate), qrp_shift_ok() the test of one shift.
+// Summing the terms in k order reproduces the serial sums exactly.
+QBX_HD void qrp_shift_term(const int32_t* Wq, const int32_t* dq, int n, int M, int k, double* rms, double* din) {
+    double mean = dq[k], var = 0.0, dn = 0.0;
+    for (int j = M; j < n; ++j) {
+        const double w = Wq[k * LDW + j];
+        mean += 0.5 * w;
+        var += 0.25 * w * w;
     }
+    for (int l = k + 1; l < M; ++l) dn += fabs((double)Wq[k * LDW + l]);
+    for (int i = 0; i <= GROUP_BITS && M + i < n; ++i) dn += fabs((double)Wq[(M + i) * LDW + k]);
+    *rms = sqrt(mean * mean + var);
+    *din = dn;
+}
+
+QBX_HD int qrp_shift_first(double S, double Din, int M) {
     // coarse units scale as 2^-sh: start just below the estimate and walk up
     const double need = (Din + 1.5 * S) / (double)(PAIR_BOUND - 64 - 4 * M);
     int sh = 0;
     while (sh < 30 && (double)(1ll << sh) < need) ++sh;
-    for (sh = sh > 1 ? sh - 1 : 0; sh <= 30; ++sh) {
-        const int64_t D = qrp_guard_D(Wq, M, L, sh);
-        if (D < 0) continue;
-        if ((double)D + 1.5 * S / (double)(1ll << sh) <= (double)PAIR_BOUND) {
+    return sh > 1 ? sh - 1 : 0;
+}
+
+// first shift tried (-1: the packed filter cannot run) and the estimate's S
+QBX_HD int qrp_shift_start(const int32_t* Wq, const int32_t* dq, int n, int M, int L, double* S_out) {
+    if (L <= M + 1) return -1;
+    double S = 0.0, Din = 0.0;
+    for (int k = 0; k < M; ++k) {
+        double r, dn;
+        qrp_shift_term(Wq, dq, n, M, k, &r, &dn);
+        S += r;
+        Din += dn;
+    }
+    *S_out = S;
+    return qrp_shift_first(S, Din, M);
+}
+
+QBX_HD bool qrp_shift_ok(const int32_t* Wq, int M, int L, int sh, double S, int64_t* D_out) {
+    const int64_t D = qrp_guard_D(Wq, M, L, sh);
+    if (D < 0) return false;
+    *D_out = D;
+    return (double)D + 1.5 * S / (double)(1ll << sh) <= (double)PAIR_BOUND;
+}
+
+QBX_HD int qrp_shift(const int32_t* Wq, const int32_t* dq, int n, int M, int L, int64_t* D_out) {
+    double S = 0.0;
+    const int sh0 = qrp_shift_start(Wq, dq, n, M, L, &S);
+    if (sh0 < 0) return -1;
+    for (int sh = sh0; sh <= 30; ++sh) {
+        int64_t D = 0;
+        if (qrp_shift_ok(Wq, M, L, sh, S, &D)) {
             *D_out = D;
             return sh;
         }
@@ -1786,7 +1926,8 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
     static_assert(OMAX > GROUP_BITS, "OMAX must cover the unrolled transition bits");
     const int O = L - M;
     LaneRP<M, OMAX> s;
-    lane_setup_rp<M, OMAX>(s, Wq, dq, n, L, B, lam);
+    wave_priority_by_progress(0u, 1u);   // the set-up at the first quarter's priority
+    lane_setup_rp<M, OMAX, true>(s, Wq, dq, n, L, B, lam);
 
     QCtx<M> x;
     x.set(W, d, dld, n, L, B, lam);
@@ -1795,6 +1936,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
     x.dlq = dlq;
 
     double bound = src.refresh();
+    src.note(bound);
     double bestE = __builtin_inf();
     uint64_t bestR = ~0ull;
     QLane q;
@@ -1857,6 +1999,7 @@ QBX_HD LaneOutF64 walk_lane_qrp(const int32_t* __restrict__ Wq, const int32_t* _
             }
             src.publish(mine);
             const double b2 = src.refresh();
+            src.note(b2);
             bound = b2 < bound ? b2 : bound;
             const double bb = bestE < bound ? bestE : bound;
             q.thrB = q_threshold(bb, scale, qm);
