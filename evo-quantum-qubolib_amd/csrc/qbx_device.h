@@ -493,6 +493,13 @@ static __device__ void finalize_problem(const InstDev& I, int lane) {
     }
 }
 
+// Commit of one chunk: the reduction key, a result slot and a candidate entry if the chunk
+// may hold the answer, then the chunk count; the last chunk finalizes.  Only candidates
+// write what the finalizer reads (slot and entry), so only they fence before counting: the
+// finalizer's acquire fence synchronizes with each candidate's release fence through the
+// read-modify-write chain on `done` (the other chunks' relaxed increments belong to it).
+// An agent-scope fence writes back and invalidates the L2 of the wave's XCD; on every
+// chunk it cost about 14 us per wave at the end of a one-tile-per-wave solve.
 __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uint64_t key,
                                             uint64_t Ebits, uint64_t rank, int lane) {
     int last = 0;
@@ -505,8 +512,8 @@ __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uin
             __threadfence();
             const uint32_t ci = atomicAdd(&I.red->cand_count, 1u);
             I.cand[ci] = tile;
+            __threadfence();
         }
-        __threadfence();
         const uint32_t done = atomicAdd(&I.red->done, 1u);
         last = (done == I.expect - 1u) ? 1 : 0;
     }
@@ -514,16 +521,26 @@ __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uin
     if (last) finalize_problem(I, lane);
 }
 
+// Tickets of a launch (its counter starts at 0, reset by qbx_prep or the server's step):
+// a workgroup's first ticket is its own index, the later ones gridDim.x + the counter's
+// value.  All workgroups start at once, and one atomic per workgroup on the counter's one
+// address queued them behind each other: at 2048 workgroups the last got its first ticket
+// 26 us after the launch.  Every launch's grid is at most its ticket count.
 __device__ __forceinline__ bool next_tile(const TileRange* __restrict__ ranges, int nranges,
                                           unsigned long long total, unsigned long long* ticket,
-                                          int lane, uint32_t& inst, uint32_t& chunk) {
+                                          int lane, bool& first, uint32_t& inst, uint32_t& chunk) {
     unsigned long long t = 0;
-    if (lane == 0) t = atomicAdd(ticket, 1ull);
-    // readfirstlane: the ticket (and so the problem, its tables and every loop bound derived
-    // from them) is wave-uniform and lives in SGPRs; W rows are then scalar loads
-    const uint32_t tlo = __builtin_amdgcn_readfirstlane((uint32_t)t);
-    const uint32_t thi = __builtin_amdgcn_readfirstlane((uint32_t)(t >> 32));
-    t = ((unsigned long long)thi << 32) | tlo;
+    if (first) {
+        t = blockIdx.x;
+        first = false;
+    } else {
+        if (lane == 0) t = (unsigned long long)gridDim.x + atomicAdd(ticket, 1ull);
+        // readfirstlane: the ticket (and so the problem, its tables and every loop bound
+        // derived from them) is wave-uniform and lives in SGPRs; W rows are then scalar loads
+        const uint32_t tlo = __builtin_amdgcn_readfirstlane((uint32_t)t);
+        const uint32_t thi = __builtin_amdgcn_readfirstlane((uint32_t)(t >> 32));
+        t = ((unsigned long long)thi << 32) | tlo;
+    }
     if (t >= total) return false;
     int lo = 0, hi = nranges - 1;
     while (lo < hi) {
@@ -588,7 +605,8 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
                                                    unsigned long long* ticket, int zero) {
     const int lane = threadIdx.x;
     uint32_t inst, chunk;
-    while (next_tile(ranges, nranges, total, ticket, lane, inst, chunk)) {
+    bool first = true;
+    while (next_tile(ranges, nranges, total, ticket, lane, first, inst, chunk)) {
         const InstDev& I = insts[inst];
         // each lane's best block over the chunk: (E, lane rank, block key)
         int32_t Eb = 0x7fffffff;
@@ -690,7 +708,8 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                                                    unsigned long long* ticket, int zero) {
     const int lane = threadIdx.x;
     uint32_t inst, chunk;
-    while (next_tile(ranges, nranges, total, ticket, lane, inst, chunk)) {
+    bool first = true;
+    while (next_tile(ranges, nranges, total, ticket, lane, first, inst, chunk)) {
         const InstDev& I = insts[inst];
         double E = __builtin_inf();
         uint64_t rank = ~0ull;
