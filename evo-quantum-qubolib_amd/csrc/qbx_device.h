@@ -534,6 +534,7 @@ __device__ __forceinline__ bool next_tile(const TileRange* __restrict__ ranges, 
         t = blockIdx.x;
         first = false;
     } else {
+        if ((unsigned long long)gridDim.x >= total) return false;   // the first tickets were all
         if (lane == 0) t = (unsigned long long)gridDim.x + atomicAdd(ticket, 1ull);
         // readfirstlane: the ticket (and so the problem, its tables and every loop bound
         // derived from them) is wave-uniform and lives in SGPRs; W rows are then scalar loads
