@@ -501,7 +501,7 @@ static __device__ void finalize_problem(const InstDev& I, int lane) {
 // An agent-scope fence writes back and invalidates the L2 of the wave's XCD; on every
 // chunk it cost about 14 us per wave at the end of a one-tile-per-wave solve.
 __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uint64_t key,
-                                            uint64_t Ebits, uint64_t rank, int lane) {
+                                            uint64_t Ebits, uint64_t rank, int lane, uint32_t count = 1u) {
     int last = 0;
     if (lane == 0) {
         const unsigned long long old = atomicMin(&I.red->minkey, (unsigned long long)key);
@@ -514,8 +514,8 @@ __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uin
             I.cand[ci] = tile;
             __threadfence();
         }
-        const uint32_t done = atomicAdd(&I.red->done, 1u);
-        last = (done == I.expect - 1u) ? 1 : 0;
+        const uint32_t done = atomicAdd(&I.red->done, count);
+        last = (done + count == I.expect) ? 1 : 0;
     }
     last = __shfl(last, 0);
     if (last) finalize_problem(I, lane);
@@ -593,9 +593,28 @@ __device__ __forceinline__ void wave_best_block(int32_t& Eb, uint64_t& lamb, uin
     Kb = uni32(Kb);
 }
 
+// The end of a wave's run of chunks of one problem: its best block (wave-uniform E, lane
+// rank, block key over the run) is re-walked once for its lowest inner rank, with the trie
+// constants read from memory, and the run commits once, counting all its chunks.
+template <int M>
+__device__ __forceinline__ void commit_run_i32(const InstDev& I, uint32_t chunk, uint32_t count, int32_t Eb,
+                                               uint64_t lamb, uint32_t Kb, int lane) {
+    uint64_t rank = ~0ull;
+    if (Eb != 0x7fffffff)
+        rank = block_rank_hi(lamb, I.L, M, Kb) |
+               block_rank_lo_i32<M>((const int32_t*)I.W, (const int32_t*)I.d, I.n, I.L, I.B, lamb, Eb, Kb,
+                                    (const int32_t*)I.delta);
+    commit_tile(I, chunk, key_of_i64(Eb), (uint64_t)(int64_t)Eb, rank, lane, count);
+}
+
 // The walk qbx_prep chose per problem: the packed pair walk (bound <= PAIR_BOUND), the
 // rebased pair walk (block-relative 16-bit energies, guarded per group), or the int32 walk.
 // All give the same lane keys; the winning block is re-walked by the int32 code either way.
+// A wave keeps the best block of its consecutive chunks of one problem and commits them
+// together when its next ticket belongs to another problem or none is left: (E, lane rank,
+// block key) orders blocks as their states' ranks, so the run's best is the best of its
+// chunks' bests.  (Per-chunk commits were three atomics per chunk: 3.1 MB of writes per
+// n = 40 launch.)
 // REC: per-reference-prefix records (I.minima).  A template flag, so that the walk without
 // records compiles exactly as if they did not exist (its kernel is at the scalar-register
 // ceiling, and a timestamp kept live across the walk costs it 3%).
@@ -607,7 +626,21 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
     const int lane = threadIdx.x;
     uint32_t inst, chunk;
     bool first = true;
-    while (next_tile(ranges, nranges, total, ticket, lane, first, inst, chunk)) {
+    // the current run (wave-uniform): its problem, last chunk, chunk count and best block
+    uint32_t run_inst = 0, run_chunk = 0, run_count = 0;
+    int32_t rEb = 0x7fffffff;
+    uint64_t rlamb = ~0ull;
+    uint32_t rKb = 0xffffffffu;
+    for (;;) {
+        const bool more = next_tile(ranges, nranges, total, ticket, lane, first, inst, chunk);
+        if (run_count != 0u && (!more || inst != run_inst)) {
+            commit_run_i32<M>(insts[run_inst], run_chunk, run_count, rEb, rlamb, rKb, lane);
+            run_count = 0u;
+            rEb = 0x7fffffff;
+            rlamb = ~0ull;
+            rKb = 0xffffffffu;
+        }
+        if (!more) break;
         const InstDev& I = insts[inst];
         // each lane's best block over the chunk: (E, lane rank, block key)
         int32_t Eb = 0x7fffffff;
@@ -619,7 +652,6 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
         const int32_t* dsrc = (const int32_t*)I.delta;
         const uint32_t t0 = chunk * I.tpc;
         const uint32_t t1 = (t0 + I.tpc < I.tiles) ? t0 + I.tpc : I.tiles;
-        uint64_t rank = ~0ull;
         const int path = I.path;
         constexpr bool rec = REC;
         if (I.status == 0 && (path == QBX_PATH_PAIR16 || path == QBX_PATH_PAIR16_REBASED)) {
@@ -643,11 +675,6 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
                 }
                 if (rec) record_prefixes(I, (uint64_t)tile * 64u, lkey, wall_clock64() - c0, lane);
             }
-            wave_best_block(Eb, lamb, Kb);
-            // the wave's winning block: one int32 re-walk for its lowest inner rank, with the
-            // trie constants read from memory
-            if (Eb != 0x7fffffff)
-                rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dsrc);
         } else {
             int32_t dl[1 << M];
             load_uniform_vgprs<1 << M>((uint32_t*)dl, (const uint32_t*)dsrc, lane & zero);
@@ -664,12 +691,16 @@ static __global__ __launch_bounds__(64) void qbx_walk_i32(const InstDev* __restr
                     if (rec) record_prefixes(I, (uint64_t)tile * 64u, lkey, wall_clock64() - c0, lane);
                 }
             }
-            wave_best_block(Eb, lamb, Kb);
-            // the wave's winning block is now uniform: re-walk it once for its lowest inner rank
-            if (Eb != 0x7fffffff)
-                rank = block_rank_hi(lamb, L, M, Kb) | block_rank_lo_i32<M>(W, d, n, L, B, lamb, Eb, Kb, dl);
         }
-        commit_tile(I, chunk, key_of_i64(Eb), (uint64_t)(int64_t)Eb, rank, lane);
+        wave_best_block(Eb, lamb, Kb);
+        if (Eb < rEb || (Eb == rEb && (lamb < rlamb || (lamb == rlamb && Kb < rKb)))) {
+            rEb = Eb;
+            rlamb = lamb;
+            rKb = Kb;
+        }
+        run_inst = inst;
+        run_chunk = chunk;
+        ++run_count;
     }
 }
 
