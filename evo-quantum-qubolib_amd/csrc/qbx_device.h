@@ -493,13 +493,15 @@ static __device__ void finalize_problem(const InstDev& I, int lane) {
     }
 }
 
-// Commit of one chunk: the reduction key, a result slot and a candidate entry if the chunk
-// may hold the answer, then the chunk count; the last chunk finalizes.  Only candidates
-// write what the finalizer reads (slot and entry), so only they fence before counting: the
-// finalizer's acquire fence synchronizes with each candidate's release fence through the
-// read-modify-write chain on `done` (the other chunks' relaxed increments belong to it).
-// An agent-scope fence writes back and invalidates the L2 of the wave's XCD; on every
-// chunk it cost about 14 us per wave at the end of a one-tile-per-wave solve.
+// Commit of one chunk, or of a wave's run of `count` chunks of one problem (its best state
+// under the chunk slot `tile`, one of the run's): the reduction key, a result slot and a
+// candidate entry if the run may hold the answer, then the chunk count; the commit that
+// completes the count finalizes.  Only candidates write what the finalizer reads (slot and
+// entry), so only they fence before counting: the finalizer's acquire fence synchronizes
+// with each candidate's release fence through the read-modify-write chain on `done` (the
+// other commits' relaxed increments belong to it).  An agent-scope fence writes back and
+// invalidates the L2 of the wave's XCD; on every chunk it cost about 14 us per wave at the
+// end of a one-tile-per-wave solve.
 __device__ __forceinline__ void commit_tile(const InstDev& I, uint32_t tile, uint64_t key,
                                             uint64_t Ebits, uint64_t rank, int lane, uint32_t count = 1u) {
     int last = 0;
