@@ -195,7 +195,9 @@ def cpu_c1() -> dict:
                      "matches_golden": bool(E == g["E"] and int(state) == g["state"])})
     wall = sum(r["wall_s"] for r in rows)
     # the product's own device='cpu' backend (libqbx_cpu.so, OpenMP) on the same instances,
-    # through the drop-in solver, end to end
+    # through the drop-in solver, end to end; best of three solves per instance (a spinning
+    # OpenMP team of one thread per core slows many-fold whenever the host runs anything
+    # else: one run measured 7.6 ms per solve against 0.4 ms)
     from qubo_lib_amd import _native
     from qubo_lib_amd.problems import Just_Q
     from qubo_lib_amd.solvers import QBF
@@ -204,9 +206,14 @@ def cpu_c1() -> dict:
     for seed in (0, 1, 2):
         Q = symmetric_int_qubo(24, seed, -100, 100, np.int32)
         QBF().solve(Just_Q(Q), device="cpu")                # warm-up (thread pool)
-        t0 = time.perf_counter()
-        sol = QBF().solve(Just_Q(Q), device="cpu")
-        wall_p = time.perf_counter() - t0
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sol = QBF().solve(Just_Q(Q), device="cpu")
+            wall_p = time.perf_counter() - t0
+            if best is None or wall_p < best[0]:
+                best = (wall_p, sol)
+        wall_p, sol = best
         g = gold[f"sym_int32_n24_s{seed}"]
         prod.append({"seed": seed, "wall_s": wall_p, "walk_ms": sol.kernel_ms,
                      "matches_golden": bool(sol.best_E == g["E"] and
@@ -218,7 +225,8 @@ def cpu_c1() -> dict:
             "wall_s": wall, "states_per_s": 3 * 2.0 ** 24 / wall, "instances": rows,
             "all_match_golden": all(r["matches_golden"] for r in rows),
             "product_cpu": {"impl": "QBF().solve(p, device='cpu'): libqbx_cpu.so, the GPU's lane walk built for "
-                                    "the host, OpenMP over lanes", "threads": HOST_THREADS,
+                                    "the host, OpenMP over lanes; best of 3 solves per instance",
+                            "threads": HOST_THREADS,
                             "wall_s": wall_p, "states_per_s": 3 * 2.0 ** 24 / wall_p, "instances": prod,
                             "all_match_golden": all(r["matches_golden"] for r in prod)}}
 
