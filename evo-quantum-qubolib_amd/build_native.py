@@ -8,7 +8,8 @@
   qbx_c_check    gcc -std=c99: a plain C caller of the C ABI, run by a GPU test
                  -> tests/_build/
 
-Every call recompiles.  Outputs are git-ignored; they travel to the GPU box with the tree.
+Every call recompiles.  Outputs are git-ignored and stay in the source tree, where the
+Python bindings and the tests load them.
 """
 from __future__ import annotations
 
@@ -54,7 +55,7 @@ def build_lib(out: Path = LIB_OUT) -> Path:
 
 
 def build_cpu(out: Path = CPU_OUT) -> Path:
-    """The device='cpu' backend: generic x86-64 code (it also runs on the GPU box's host)."""
+    """The device='cpu' backend: generic x86-64 code (no -march: it runs on any x86-64 host)."""
     out.parent.mkdir(parents=True, exist_ok=True)
     tmp = out.with_suffix(".so.tmp")
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wno-unknown-pragmas",

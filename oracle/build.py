@@ -17,7 +17,7 @@ OUT = HERE / "_build" / "liboracle.so"
 
 def build(out: Path = OUT) -> Path:
     out.parent.mkdir(parents=True, exist_ok=True)
-    # generic x86-64 code: the same build also runs on the GPU box's host CPU
+    # generic x86-64 code: the same build runs on any x86-64 host
     cmd = ["gcc", "-O3", "-fopenmp", "-fPIC", "-shared", "-o", str(out), str(HERE / "qbf_oracle.c")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
