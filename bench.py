@@ -668,8 +668,8 @@ def copy_calibration(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["exhaustive", "float", "maxcut", "batch", "serve", "copy"],
                     default="exhaustive")
     ap.add_argument("--headline-only", action="store_true",
