@@ -15,7 +15,8 @@ metric's second clause and the other configurations, each timed in this run:
             continuous-batching server, against sequential plain solves of the same
             arrivals: solves/s, p50/p99 latency, identical answers
   cpu_c1    the reference's CPU configuration: QBF (numpy restatement) on n = 24 int32
-            RandomQUBO-style instances, seeds 0-2, end to end, exact against the goldens
+            RandomQUBO-style instances, seeds 0-2, end to end, exact against the goldens;
+            beside it the product's own device='cpu' backend on the same instances
   float34   n = 34 float32 N(0,1) instance, FP64-exact argmin: states/s, |E - E_fp64|,
             argmin agreement with the FP64 brute-force golden
 
