@@ -20,8 +20,11 @@ metric's second clause and the other configurations, each timed in this run:
   float34   n = 34 float32 N(0,1) instance, FP64-exact argmin: states/s, |E - E_fp64|,
             argmin agreement with the FP64 brute-force golden
 
-    python bench.py [--steps K] [--warmup W] [--workload exhaustive|float|maxcut|batch|serve|copy]
+    python bench.py [--steps K] [--warmup W] [--workload exhaustive|float|maxcut|batch|serve|copy|float-ties]
                     [--headline-only] [--no-cpu-baseline]
+
+--workload float-ties times both float filters on inputs whose ties FP64 keeps only to
+rounding (MaxCut QUBOs scaled by 0.1 and 1/3, a float32 TSP), one row per instance.
 
 `value` is states/s for the whole job (one GPU) with Q resident in device memory (HIP-event
 time of each solve's kernels); `host_inclusive` gives the wall-clock rate with the
@@ -286,6 +289,38 @@ def float_leg(args, devinfo, n: int = 34, steps: int = 20, warmup: int = 3) -> d
             "parity": parity,
             "roofline": roofline(value, r.info["walk"], r.info["inner_bits"], devinfo,
                                  "float" if n == 34 and args.seed == 0 else None)}
+
+
+def float_ties(args, devinfo) -> dict:
+    """Float inputs with ties in real arithmetic that FP64 keeps only to rounding (MaxCut
+    G(n, 1/2) QUBOs scaled by 0.1 and by 1/3, off any power-of-two lattice; a float32 TSP
+    of 6 cities, n = 36): device time per solve of the packed and the int32 float filter,
+    best of --steps, and whether they agree (the default filter for a lone problem of
+    n >= 28 is the packed one)."""
+    from qubo_lib_amd import engine
+    from qubo_lib_amd.problems.generators import maxcut_gnp
+    from qubo_lib_amd.problems.tsp import TSP
+    cases = []
+    for n in (34, 36, 40):
+        Q = maxcut_gnp(n, 0.5, seed=args.seed + 1).astype(np.float64)
+        cases += [(f"maxcut G({n},1/2) x 0.1", Q * 0.1), (f"maxcut G({n},1/2) / 3", Q / 3)]
+    cases.append(("TSP 6 cities float32 (seed 42)", TSP.init_random_instance(seed=42, n_cities=6).build_Q_matrix()))
+    rows = []
+    for name, Q in cases:
+        p = engine.prepare(Q)
+        row = {"instance": name, "n": p.n}
+        answers = set()
+        for f in ("packed", "int32"):
+            r, kms, _ = _time_solves(p, max(args.steps, 1), max(args.warmup, 1), {"float_filter": f})
+            row[f"{f}_kernel_ms"] = float(np.min(kms))
+            answers.add((r.E, r.state))
+        row["same_answer"] = len(answers) == 1
+        row["E"] = r.E
+        log(json.dumps(row))
+        rows.append(row)
+    return {"metric": "float filter device time per solve on tie-heavy inputs", "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "data": "synthetic", "dtype": "f64", "config": {"workload": "float-ties", "rows": rows}}
 
 
 def exhaustive(args, ctx, devinfo):
@@ -671,7 +706,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["exhaustive", "float", "maxcut", "batch", "serve", "copy"],
+    ap.add_argument("--workload", choices=["exhaustive", "float", "maxcut", "batch", "serve", "copy",
+                                          "float-ties"],
                     default="exhaustive")
     ap.add_argument("--headline-only", action="store_true",
                     help="exhaustive workload: skip the batch / serve / cpu_c1 / float34 legs")
@@ -705,6 +741,8 @@ def main():
         line = batch(args, ctx, devinfo)
     elif args.workload == "serve":
         line = serve(args, ctx, devinfo)
+    elif args.workload == "float-ties":
+        line = float_ties(args, devinfo)
     else:
         line = exhaustive(args, ctx, devinfo)
         if args.workload == "exhaustive" and not args.headline_only:
