@@ -272,8 +272,11 @@ def _time_solves(prep, steps, warmup, kw):
     return r, kms, wall
 
 
-def float_leg(args, devinfo, n: int = 34, steps: int = 20, warmup: int = 3) -> dict:
-    """The float configuration: n = 34 N(0,1) float32 instance, FP64-exact answer."""
+def float_leg(args, devinfo, n: int = 34, steps: int = 500, warmup: int = 50) -> dict:
+    """The float configuration: n = 34 N(0,1) float32 instance, FP64-exact answer.  A solve
+    takes half a millisecond, so the leg times 500 of them (a quarter second) after 50
+    untimed ones: over 20 steps after 3 the GPU's clock is still rising (0.51 against 0.48
+    ms per solve measured)."""
     from qubo_lib_amd import engine
     from qubo_lib_amd.problems.generators import symmetric_normal_qubo
     Q = symmetric_normal_qubo(n, args.seed)

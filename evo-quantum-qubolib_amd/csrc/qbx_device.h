@@ -183,7 +183,7 @@ static __global__ __launch_bounds__(64 * SEED_WAVES) void qbx_seed(InstDev* inst
         v += (b < c && ((xb >> b) & 1ull)) ? w : 0.0;
     }
     v = (c < n && ((xb >> c) & 1ull)) ? v + d[c] : 0.0;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = 32; o > 0; o >>= 1) v += wave_xor(v, o);
     if (c == 0 && v - v == 0.0) I.red->minkey = key_of_f64(v + I.fbound * 0x1p-40);
 }
 
@@ -435,7 +435,7 @@ __device__ __forceinline__ void record_prefixes(const InstDev& I, uint64_t lam0,
                                                 int lane) {
     const int gw = I.mshift < 6 ? I.mshift : 6;
     for (int s = 0; s < gw; ++s) {
-        const uint64_t k2 = __shfl_xor(key, 1 << s);
+        const uint64_t k2 = wave_xor(key, 1 << s);
         key = k2 < key ? k2 : key;
     }
     if ((lane & ((1 << gw) - 1)) == 0 && key != ~0ull) {
@@ -724,7 +724,7 @@ struct WaveBound {
     __device__ void publish(double e) {
         if (!live) return;
         for (int off = 32; off > 0; off >>= 1) {
-            const double e2 = __shfl_xor(e, off);
+            const double e2 = wave_xor(e, off);
             e = e2 < e ? e2 : e;
         }
         if (e < offered) {
@@ -799,8 +799,8 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
-            const double E2 = __shfl_xor(E, off);
-            const uint64_t r2 = __shfl_xor(rank, off);
+            const double E2 = wave_xor(E, off);
+            const uint64_t r2 = wave_xor(rank, off);
             if (E2 < E || (E2 == E && r2 < rank)) { E = E2; rank = r2; }
         }
         union { double d; uint64_t u; } cv;

@@ -60,6 +60,33 @@ namespace qbx {
 #define QBX_CONST_AS
 #endif
 
+#if defined(__HIPCC__) || defined(__HIP__)
+// v from lane (this lane XOR off), 0 < off < 64: one ds_bpermute per 32 bits.  __shfl_xor
+// also tests that the partner lies inside its width, a per-lane bound that the compiler
+// keeps live across the whole float walk (in scratch); in a wave of 64 the partner always
+// exists.
+template <class T>
+__device__ __forceinline__ T wave_xor(T v, int off) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    const int a = (int)((__lane_id() ^ (unsigned)off) << 2);
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(a, __builtin_bit_cast(int, v)));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)(uint32_t)u);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)(uint32_t)(u >> 32));
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+}
+// v of lane l, l wave-uniform
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+#endif
+
 // Path counters of the host build (the CPU tests check that every path of the float
 // filter is exercised); compiled out of the device build.
 #ifdef QBX_EMU_STATS
@@ -497,7 +524,10 @@ QBX_HD void lane_setup_i32(LaneI32<M, OMAX>& s, const int32_t* __restrict__ W, c
         const uint32_t v = (c < n ? (uint32_t)d[c] : 0u) + acc;
         const bool in_hi = c >= hcol && c < n && ((p_hi >> (c - hcol)) & 1ull);
         uint32_t e = in_hi ? v : 0u;
-        for (int o = 32; o > 0; o >>= 1) e += (uint32_t)__shfl_xor((int)e, o);
+        // (the float walks exchange without __shfl_xor's width test: see wave_xor; the
+        // integer walks' register allocation is best left as it is with the test)
+        for (int o = 32; o > 0; o >>= 1)
+            e += SROWS ? (uint32_t)wave_xor((int)e, o) : (uint32_t)__shfl_xor((int)e, o);
         Ec = (uint32_t)__builtin_amdgcn_readfirstlane((int)e);
 #pragma unroll
         for (int k = 0; k < M; ++k) fc[k] = __builtin_amdgcn_readlane((int)v, k);
@@ -1230,7 +1260,8 @@ QBX_HD double q_scale(double bound, int n) {
 QBX_HD double q_margin(int n) { return 0.25 * (double)n * (double)(n + 1) + 1.0; }
 // int32 threshold for "can hold a state with E <= b"
 QBX_HD int32_t q_threshold(double b, double scale, double qm) {
-    if (!(b < 1.0e300)) return 0x7fffffff;
+    // no threshold for a NaN; +inf (the empty key's energy) gives t = +inf below
+    if (b != b) return 0x7fffffff;
     const double t = floor(b * scale + qm);
     if (t >= 2147483647.0) return 0x7fffffff;
     if (t <= -2147483648.0) return (int32_t)0x80000000;
@@ -1409,7 +1440,7 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
 }
 __device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
     for (int o = 32; o > 0; o >>= 1) {
-        const int32_t w = __shfl_xor(v, o);
+        const int32_t w = wave_xor(v, o);
         v = w < v ? w : v;
     }
     return v;
@@ -1551,7 +1582,7 @@ QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uin
                     // the int32 filter's exact test: the block's least quantized energy
                     const bool set = lane >= M && lane < x.n && ((hs >> (lane - M)) & 1ull);
                     uint32_t e0 = set ? colq : 0u;
-                    for (int o = 32; o > 0; o >>= 1) e0 += (uint32_t)__shfl_xor((int32_t)e0, o);
+                    for (int o = 32; o > 0; o >>= 1) e0 += (uint32_t)wave_xor((int32_t)e0, o);
                     uint32_t fq[M];
 #pragma unroll
                     for (int j = 0; j < M; ++j) fq[j] = readlane_u32(colq, j);
@@ -1574,10 +1605,10 @@ QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uin
                 // the block in FP64: its states' energies and ranks, the wave's least pair
                 const bool set = lane >= M && lane < x.n && ((hs >> (lane - M)) & 1ull);
                 double e0 = set ? col : 0.0;
-                for (int o = 32; o > 0; o >>= 1) e0 += __shfl_xor(e0, o);
+                for (int o = 32; o > 0; o >>= 1) e0 += wave_xor(e0, o);
                 double f[M];
 #pragma unroll
-                for (int j = 0; j < M; ++j) f[j] = __shfl(col, j);
+                for (int j = 0; j < M; ++j) f[j] = readlane_f64(col, j);
                 const uint64_t rblk = laneRank | ((uint64_t)block_key(cf, SBk, pmaskS) << M);
                 const int par = parp ^ parity32(gc);
                 double E = __builtin_inf();
@@ -1592,8 +1623,8 @@ QBX_HD void q_flush_wave(QBuf& b, QLane& q, const QCtx<M>& x, double& bestE, uin
                     }
                 }
                 for (int o = 32; o > 0; o >>= 1) {
-                    const double E2 = __shfl_xor(E, o);
-                    const uint64_t r2 = __shfl_xor(r, o);
+                    const double E2 = wave_xor(E, o);
+                    const uint64_t r2 = wave_xor(r, o);
                     if (E2 < E || (E2 == E && r2 < r)) { E = E2; r = r2; }
                 }
                 if (E < bestE || (E == bestE && r < bestR)) { bestE = E; bestR = r; }
@@ -1866,15 +1897,17 @@ QBX_HD int32_t clamp_i32(int64_t v) {
 template <int M, int OMAX>
 QBX_HD void qrp_record(const LaneRP<M, OMAX>& s, uint32_t c, const int32_t* m, const int32_t* t, int32_t bias,
                        int sh, QLane& q, QBuf& b) {
+    // at least one half passes (the caller's test); lb starts from the first that does, so
+    // that no 64-bit sentinel stays live across the walk (the compiler kept one in scratch)
     uint32_t mask = 0;
-    int64_t lb = 0x7fffffffffffffffLL;
+    int64_t lb = 0;
     for (int h = 0; h < 2; ++h) {
         if (m[h] > t[h]) continue;
-        mask |= 1u << h;
         const int64_t base = (int64_t)s.E0 + (h ? (int64_t)s.up : 0);
         const int64_t lo = base + ((int64_t)(m[h] - bias - QRP_ERR) << sh);
         const int32_t ub = clamp_i32(base + ((int64_t)(m[h] - bias + QRP_ERR) << sh));
-        lb = lo < lb ? lo : lb;
+        lb = (mask == 0u || lo < lb) ? lo : lb;
+        mask |= 1u << h;
         q.laneMin = ub < q.laneMin ? ub : q.laneMin;
     }
     q_update_thr(q);
