@@ -770,7 +770,11 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                 const uint32_t* Wpc = (const uint32_t*)(Wq + (size_t)n * LDW);
                 const uint32_t* psrc = (const uint32_t*)(dsrc + (1 << M));
                 uint32_t dlp[1 << M];
-                load_uniform_vgprs<1 << M>(dlp, psrc, lane & zero);
+                // the walk without records makes its opaque zero here: kept from the kernel's
+                // start it costs a scratch slot (the recording walk allocates best as it was)
+                uint32_t z = (uint32_t)(lane & zero);
+                if constexpr (!REC) QBX_OPAQUE(z);
+                load_uniform_vgprs<1 << M>(dlp, psrc, z);
                 for (uint32_t tile = t0; tile < t1; ++tile) {
                     const uint64_t lam0 = (uint64_t)tile * 64u;
                     const uint64_t lam = lam0 + (uint64_t)lane;
@@ -784,7 +788,9 @@ static __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) 
                 }
             } else {
                 int32_t dlq[1 << M];
-                load_uniform_vgprs<1 << M>((uint32_t*)dlq, (const uint32_t*)dsrc, lane & zero);
+                uint32_t z = (uint32_t)(lane & zero);
+                if constexpr (!REC) QBX_OPAQUE(z);
+                load_uniform_vgprs<1 << M>((uint32_t*)dlq, (const uint32_t*)dsrc, z);
                 for (uint32_t tile = t0; tile < t1; ++tile) {
                     const uint64_t lam0 = (uint64_t)tile * 64u;
                     const uint64_t lam = lam0 + (uint64_t)lane;

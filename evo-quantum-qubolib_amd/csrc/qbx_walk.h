@@ -1453,6 +1453,7 @@ __device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
 template <class T, class Acc, int M>
 __device__ __forceinline__ Acc block_column_wave(const T* __restrict__ W, const T* __restrict__ d, int n,
                                                  uint64_t hs, int lane) {
+    QBX_OPAQUE(lane);   // the column offsets from here, not kept from the kernel's start
     Acc acc = lane < n ? (Acc)d[lane] : (Acc)0;
     const int lim = lane < M ? n : lane;
     // batches of 8 unconditional row loads (rows clamped into the table), then the adds
@@ -1478,6 +1479,7 @@ template <int M>
 __device__ __forceinline__ void block_columns_wave(const int32_t* __restrict__ Wq, const int32_t* __restrict__ dq,
                                                    const double* __restrict__ W, const double* __restrict__ d,
                                                    int n, uint64_t hs, int lane, uint32_t& colq, double& col) {
+    QBX_OPAQUE(lane);   // the column offsets from here, not kept from the kernel's start (in scratch)
     uint32_t aq = lane < n ? (uint32_t)dq[lane] : 0u;
     double ad = lane < n ? d[lane] : 0.0;
     const int lim = lane < M ? n : lane;
