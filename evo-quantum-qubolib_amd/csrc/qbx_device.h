@@ -238,6 +238,27 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             dlp[t] = pack_both((uint32_t)dl[t]);
         }
         red_i[tid] = acc;
+        // the rebased walk's guard constant and field estimate (rp_guard_D, rp_worth_it):
+        // inner bit k's share on thread k, from the matrix in LDS, beside the reduction
+        // (one thread reading the tables back from memory took most of this kernel's time)
+        __shared__ long long sh_Dk[8];
+        __shared__ double sh_Sk[8];
+        if (tid < M) {
+            const int k = tid;
+            long long Dk = 0;
+            for (int l = k + 1; l < M; ++l) Dk += llabs(w_elem<int32_t, long long>(Q, n, k, l));
+            Dk += llabs(w_elem<int32_t, long long>(Q, n, M, k));
+            for (int i = 0; i < GROUP_BITS && M + 1 + i < I.L; ++i)
+                Dk += llabs(w_elem<int32_t, long long>(Q, n, M + 1 + i, k));
+            double mean = (double)(int32_t)d_elem<int32_t, long long>(Q, n, k), var = 0.0;
+            for (int j = M; j < n; ++j) {
+                const double w = (double)(int32_t)w_elem<int32_t, long long>(Q, n, k, j);
+                mean += 0.5 * w;
+                var += 0.25 * w * w;
+            }
+            sh_Dk[k] = Dk;
+            sh_Sk[k] = sqrt(mean * mean + var);
+        }
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
             if (tid < s) red_i[tid] += red_i[tid + s];
@@ -252,8 +273,13 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             if (I.L > M && red_i[0] <= PAIR_BOUND) {
                 I.path = QBX_PATH_PAIR16;
             } else if (I.status == 0 && I.L > M) {
-                const int64_t D = rp_guard_D(W, M, I.L);
-                if (rp_worth_it(W, d, n, M, I.L, D)) {
+                int64_t D = 0;
+                double S = 0.0;
+                for (int k = 0; k < M; ++k) {
+                    D += sh_Dk[k];
+                    S += sh_Sk[k];
+                }
+                if (D < PAIR_BOUND && (double)D + 1.5 * S <= (double)PAIR_BOUND) {
                     I.path = QBX_PATH_PAIR16_REBASED;
                     I.slack = (int32_t)(PAIR_BOUND - D);
                 }
