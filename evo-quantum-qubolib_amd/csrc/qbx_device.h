@@ -208,9 +208,25 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
     __shared__ long long red_i[256];
     __shared__ uint32_t q_lds[MAX_N * MAX_N * 2];
     {
+        // batches of 8 loads per thread, all in flight before the first LDS write (one at a
+        // time, each load waited for its memory round trip: most of this kernel's time, and
+        // more when the matrix lies in mapped host memory); indices clamped into the matrix
         const uint32_t* src = (const uint32_t*)I.Q;
         const int words = n * n * (I.kind == KIND_I32 ? 1 : 2);
-        for (int w = tid; w < words; w += blockDim.x) q_lds[w] = src[w];
+        const int step = (int)blockDim.x;
+        for (int w0 = tid; w0 < words; w0 += 8 * step) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int w = w0 + u * step;
+                v[u] = src[w < words ? w : words - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int w = w0 + u * step;
+                if (w < words) q_lds[w] = v[u];
+            }
+        }
         __syncthreads();
     }
     if (I.kind == KIND_I32) {
@@ -321,15 +337,19 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         const bool ok = fb <= 1.0e300;
         const double scale = ok ? q_scale(fb, n) : 1.0;
         __syncthreads();
+        // the quantized table also into LDS, where the steps below read it (W and d are
+        // recomputed from the matrix in LDS: no read of what this kernel just wrote to memory)
+        __shared__ int32_t wq_lds[MAX_N * LDW];
         long long qacc = 0;
         for (int idx = tid; idx < n * LDW; idx += blockDim.x) {
             const int a = idx / LDW, b = idx - a * LDW;
-            const int32_t q = ok ? (int32_t)rint(W[idx] * scale) : 0;
+            const int32_t q = ok ? (int32_t)rint(w_elem<double, double>(Q, n, a, b) * scale) : 0;
             Wq[idx] = q;
+            wq_lds[idx] = q;
             if (b > a && b < n) qacc += q < 0 ? -(long long)q : q;
         }
         for (int a = tid; a < 64; a += blockDim.x) {
-            const int32_t q = ok ? (int32_t)rint(d[a] * scale) : 0;
+            const int32_t q = ok ? (int32_t)rint((a < n ? d_elem<double, double>(Q, n, a) : 0.0) * scale) : 0;
             dq[a] = q;
             qacc += q < 0 ? -(long long)q : q;
         }
@@ -341,21 +361,18 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         }
         // trie constants of the filter from the quantized couplings (exact int32 sums)
         for (uint32_t t = tid; t < (1u << M); t += blockDim.x)
-            dlq[t] = (int32_t)trie_from_table<int32_t, long long>(Wq, M, t);
-        // the rows the packed filter's shift choice reads (inner, pair and group bits), in
-        // LDS; thread 0 takes the estimate, then one thread per candidate shift tests it
-        __shared__ int32_t wq_head[(7 + 1 + GROUP_BITS) * LDW];
+            dlq[t] = (int32_t)trie_from_table<int32_t, long long>(wq_lds, M, t);
+        // the packed filter's shift choice (it reads the inner, pair and group bits' rows):
+        // thread 0 takes the estimate, then one thread per candidate shift tests it
         __shared__ double sh_S, sh_rms[8], sh_din[8];
         __shared__ int sh_start;
         __shared__ int sh_ok[32];
         __shared__ long long sh_D[32];
-        const int head_rows = n < M + 1 + GROUP_BITS ? n : M + 1 + GROUP_BITS;
-        for (int idx = tid; idx < head_rows * LDW; idx += blockDim.x) wq_head[idx] = Wq[idx];
         // finite energies whose quantized bound fits int32 (q_scale guarantees it)
         const bool in_range = ok && red_i[0] <= 0x7ffffffeLL;
         const bool try_packed = in_range && !(I.flags & QBX_FLAG_FLOAT_INT32_FILTER) && I.L > M + 1;
         __syncthreads();
-        if (try_packed && tid < M) qrp_shift_term(wq_head, dq, n, M, tid, &sh_rms[tid], &sh_din[tid]);
+        if (try_packed && tid < M) qrp_shift_term(wq_lds, dq, n, M, tid, &sh_rms[tid], &sh_din[tid]);
         __syncthreads();
         if (tid == 0) {
             double S = 0.0, Din = 0.0;
@@ -370,7 +387,7 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
         const int s0 = sh_start;
         if (tid < 31) {
             int64_t D = 0;
-            sh_ok[tid] = (s0 >= 0 && tid >= s0 && qrp_shift_ok(wq_head, M, I.L, tid, sh_S, &D)) ? 1 : 0;
+            sh_ok[tid] = (s0 >= 0 && tid >= s0 && qrp_shift_ok(wq_lds, M, I.L, tid, sh_S, &D)) ? 1 : 0;
             sh_D[tid] = D;
         }
         __syncthreads();
@@ -400,7 +417,7 @@ static __global__ __launch_bounds__(256) void qbx_prep(InstDev* insts, const uin
             const int sh = I.qshift;
             uint32_t* Wpc = (uint32_t*)(Wq + (size_t)n * LDW);
             uint32_t* dlpc = (uint32_t*)(dlq + (1 << M));
-            for (int idx = tid; idx < n * LDW; idx += blockDim.x) Wpc[idx] = pack_both((uint32_t)rnd_shr(Wq[idx], sh));
+            for (int idx = tid; idx < n * LDW; idx += blockDim.x) Wpc[idx] = pack_both((uint32_t)rnd_shr(wq_lds[idx], sh));
             for (uint32_t t = tid; t < (1u << M); t += blockDim.x) dlpc[t] = pack_both((uint32_t)rnd_shr(dlq[t], sh));
         }
     }
