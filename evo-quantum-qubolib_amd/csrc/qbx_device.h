@@ -133,9 +133,25 @@ static __global__ __launch_bounds__(64 * SEED_WAVES) void qbx_seed(InstDev* inst
     const int c = (int)__lane_id();            // this lane's variable (bit position c)
     const int wv = (int)(threadIdx.x >> 6);
     __shared__ float wf[MAX_N * 64];           // w_ab in FP32 (row a, column b), d_a on the diagonal
-    for (int i = threadIdx.x; i < MAX_N * 64; i += blockDim.x) {
-        const int a = i >> 6, b = i & 63;
-        wf[i] = (a < n && b < n) ? (float)(a == b ? d[a] : W[a * LDW + b]) : 0.0f;
+    __shared__ double wd[MAX_N * 64];          // the same in FP64, for the exact energy at the end
+    {
+        // every load of a thread in flight before its first LDS write
+        constexpr int PER = (MAX_N * 64 + 64 * SEED_WAVES - 1) / (64 * SEED_WAVES);
+        double v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = (int)threadIdx.x + u * 64 * SEED_WAVES;
+            const int a = i >> 6, b = i & 63;
+            v[u] = (i < MAX_N * 64 && a < n && b < n) ? (a == b ? d[a] : W[a * LDW + b]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = (int)threadIdx.x + u * 64 * SEED_WAVES;
+            if (i < MAX_N * 64) {
+                wf[i] = (float)v[u];
+                wd[i] = v[u];
+            }
+        }
     }
     __syncthreads();
     // the descent of this wave: x wave-uniform, lane c holds g_c, the energy change of
@@ -168,7 +184,7 @@ static __global__ __launch_bounds__(64 * SEED_WAVES) void qbx_seed(InstDev* inst
     __syncthreads();
     if (wv != 0) return;
     // the least of the descents, then its exact energy: lane c adds x_c (d_c + sum_{b < c,
-    // x_b} W_bc), reading column c of row b (coalesced)
+    // x_b} W_bc), from the FP64 copy in LDS
     float eb = best_E[0];
     uint64_t xb = best_x[0];
     for (int k = 1; k < SEED_WAVES; ++k)
@@ -177,12 +193,12 @@ static __global__ __launch_bounds__(64 * SEED_WAVES) void qbx_seed(InstDev* inst
             xb = best_x[k];
         }
     double v = 0.0;
-#pragma unroll 8
-    for (int b = 0; b < n; ++b) {      // same trip count on every lane: the loads pipeline
-        const double w = W[b * LDW + c];
+    for (int b = 0; b < n; ++b) {
+        const double w = wd[b * 64 + c];
         v += (b < c && ((xb >> b) & 1ull)) ? w : 0.0;
     }
-    v = (c < n && ((xb >> c) & 1ull)) ? v + d[c] : 0.0;
+    const int cd = c < n ? c : 0;       // the diagonal entry d_c (index kept inside the table)
+    v = (c < n && ((xb >> c) & 1ull)) ? v + wd[cd * 65] : 0.0;
     for (int o = 32; o > 0; o >>= 1) v += wave_xor(v, o);
     if (c == 0 && v - v == 0.0) I.red->minkey = key_of_f64(v + I.fbound * 0x1p-40);
 }
